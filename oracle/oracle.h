@@ -1,0 +1,53 @@
+/* Test infrastructure: CPU restatement of the reference BSSN RHS
+ * (bssn/src/rhs.cpp:80-388 with bssn/src/derivs.cpp stencils).
+ *
+ * Only the tests, the smoke check and the CPU-baseline leg of the benchmark
+ * load this library, and only as the checker / the timed CPU path.  The
+ * product (libsympygr_amd.so) never links it.
+ */
+#ifndef SYMPYGR_ORACLE_H
+#define SYMPYGR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physics parameters read by the reference bssnrhs from bssn:: globals
+ * (bssn/src/parameters.cpp:99-109, rhs.cpp:148-151,215-219,330). */
+typedef struct {
+    uint32_t lambda[4];      /* BSSN_LAMBDA (unsigned in the reference) */
+    double lambda_f[2];      /* BSSN_LAMBDA_F */
+    double eta_const;        /* ETA_CONST */
+    double eta_r0;           /* ETA_R0 */
+    double eta_damping_exp;  /* ETA_DAMPING_EXP */
+    double ko_sigma;         /* KO_DISS_SIGMA */
+    int32_t upwind;          /* 0: advective derivs = centred deriv64 (as shipped)
+                                1: 4th-order shift-upwinded deriv42adv */
+} oracle_params;
+
+/* One block, same argument meaning as the reference bssnrhs (rhs.h:33-36). */
+int oracle_bssnrhs(double **out, const double **in, uint64_t offset,
+                   const double *pmin, const double *pmax, const uint32_t *sz,
+                   uint32_t bflag, const oracle_params *p);
+
+/* Many blocks; block b uses offsets[b], pmin/pmax[3b..3b+2], sz[3b..3b+2],
+ * bflag[b].  nthreads > 0 runs an OpenMP parallel-for over blocks. */
+int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t nblocks,
+                          const uint64_t *offsets, const double *pmin,
+                          const double *pmax, const uint32_t *sz,
+                          const uint32_t *bflag, const oracle_params *p,
+                          int nthreads);
+
+/* One stencil sweep: which = 0..2 first x/y/z, 3..5 second xx/yy/zz,
+ * 6..8 advective x/y/z (needs beta_d), 9..11 Kreiss-Oliger x/y/z.
+ * upwind selects the advective stencil as in oracle_params. */
+int oracle_deriv(int which, double *out, const double *u, double h,
+                 const uint32_t *sz, const double *beta, uint32_t bflag,
+                 int upwind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

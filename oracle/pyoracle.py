@@ -1,0 +1,163 @@
+"""Test infrastructure: ctypes access to the CPU oracle (liboracle.so) and,
+where it has been built, the compiled reference (_ref/libbssnref_{c,u}.so).
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P_D = ctypes.POINTER(ctypes.c_double)
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("lambda_", ctypes.c_uint32 * 4),
+        ("lambda_f", ctypes.c_double * 2),
+        ("eta_const", ctypes.c_double),
+        ("eta_r0", ctypes.c_double),
+        ("eta_damping_exp", ctypes.c_double),
+        ("ko_sigma", ctypes.c_double),
+        ("upwind", ctypes.c_int32),
+    ]
+
+
+def make_params(upwind=True, ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
+                lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0)):
+    p = OracleParams()
+    for i in range(4):
+        p.lambda_[i] = lambda_[i]
+    p.lambda_f[0], p.lambda_f[1] = lambda_f
+    p.eta_const, p.eta_r0, p.eta_damping_exp = eta_const, eta_r0, eta_damping_exp
+    p.ko_sigma = ko_sigma
+    p.upwind = 1 if upwind else 0
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(path)
+        L.oracle_bssnrhs.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(OracleParams)]
+        L.oracle_bssnrhs_blocks.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint32,
+                                            ctypes.POINTER(ctypes.c_uint64), P_D, P_D,
+                                            ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(OracleParams), ctypes.c_int]
+        L.oracle_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), P_D,
+                                   ctypes.c_uint32, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptrs(arr):
+    return (P_D * arr.shape[0])(*[arr[f].ctypes.data_as(P_D) for f in range(arr.shape[0])])
+
+
+def rhs_block(u, sz, pmin, pmax, bflag, params, out=None):
+    """Oracle RHS of one block. u: (24, npts) float64 C-contiguous."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if out is None:
+        out = np.zeros_like(u)
+    rc = lib().oracle_bssnrhs(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+                              (ctypes.c_uint32 * 3)(*sz), bflag, ctypes.byref(params))
+    if rc != 0:
+        raise RuntimeError(f"oracle_bssnrhs failed: {rc}")
+    return out
+
+
+def rhs_blocks(u, bl, params, nthreads=1, out=None):
+    """Oracle RHS over a BlockList (u: (24, bl.total))."""
+    nb = len(bl)
+    if out is None:
+        out = np.zeros_like(u)
+    offs = (ctypes.c_uint64 * nb)(*bl.offsets)
+    pmin = (ctypes.c_double * (3 * nb))(*[v for p in bl.pmin for v in p])
+    pmax = (ctypes.c_double * (3 * nb))(*[v for p in bl.pmax for v in p])
+    sz = (ctypes.c_uint32 * (3 * nb))(*[v for s in bl.sz for v in s])
+    bf = (ctypes.c_uint32 * nb)(*bl.bflag)
+    rc = lib().oracle_bssnrhs_blocks(_ptrs(out), _ptrs(u), nb, offs, pmin, pmax, sz, bf, ctypes.byref(params), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_bssnrhs_blocks failed: {rc}")
+    return out
+
+
+def deriv(which, u, h, sz, bflag, beta=None, upwind=True):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64).ctypes.data_as(P_D)
+    rc = lib().oracle_deriv(which, out.ctypes.data_as(P_D), u.ctypes.data_as(P_D), h, (ctypes.c_uint32 * 3)(*sz),
+                            b, bflag, 1 if upwind else 0)
+    if rc != 0:
+        raise RuntimeError("oracle_deriv failed")
+    return out
+
+
+# ---------------------------------------------------------------- reference
+def ref_path(variant):
+    return os.path.join(HERE, "_ref", f"libbssnref_{variant}.so")
+
+
+def ref_available(variant="u"):
+    return os.path.exists(ref_path(variant))
+
+
+_ref = {}
+
+
+def ref_lib(variant):
+    if variant not in _ref:
+        L = ctypes.CDLL(ref_path(variant))
+        L.ref_bssnrhs.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint, P_D, P_D,
+                                  ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.POINTER(ctypes.c_uint),
+                                  P_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ref_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint), P_D,
+                                ctypes.c_uint]
+        L.ref_fake_initial_data.argtypes = [ctypes.c_double] * 3 + [P_D]
+        _ref[variant] = L
+    return _ref[variant]
+
+
+def ref_rhs_block(variant, u, sz, pmin, pmax, bflag, params):
+    L = ref_lib(variant)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    lam = (ctypes.c_uint * 4)(*params.lambda_)
+    lf = (ctypes.c_double * 2)(*params.lambda_f)
+    L.ref_bssnrhs(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+                  (ctypes.c_uint * 3)(*sz), bflag, lam, lf, params.eta_const, params.eta_r0,
+                  params.eta_damping_exp, params.ko_sigma)
+    return out
+
+
+def ref_deriv(variant, which, u, h, sz, bflag, beta=None):
+    L = ref_lib(variant)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros_like(u)
+    b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64).ctypes.data_as(P_D)
+    L.ref_deriv(which, out.ctypes.data_as(P_D), u.ctypes.data_as(P_D), h, (ctypes.c_uint * 3)(*sz), b, bflag)
+    return out
+
+
+def ref_fake_initial_data(variant, sz, pmin, pmax):
+    L = ref_lib(variant)
+    n = sz[0] * sz[1] * sz[2]
+    u = np.zeros((24, n))
+    h = [(pmax[d] - pmin[d]) / (sz[d] - 1) for d in range(3)]
+    v = (ctypes.c_double * 24)()
+    for k in range(sz[2]):
+        for j in range(sz[1]):
+            for i in range(sz[0]):
+                L.ref_fake_initial_data(pmin[0] + i * h[0], pmin[1] + j * h[1], pmin[2] + k * h[2], v)
+                u[:, i + sz[0] * (j + sz[1] * k)] = np.frombuffer(v, dtype=np.float64)
+    return u

@@ -1,0 +1,61 @@
+// Test infrastructure only: a C entry point into the compiled reference
+// `bssnrhs` (bssn/src/rhs.cpp:80-388) so the fixture generator can call it
+// through ctypes.  This file is ours; it is linked with the reference's own
+// sources by oracle/Makefile and is never part of the product library.
+#include "rhs.h"
+#include "derivs.h"
+#include "parameters.h"
+
+extern "C" {
+
+// Parameters are the reference's mutable globals (bssn/src/parameters.cpp:99-109);
+// they are set here before every call, exactly as a Dendro driver would.
+int ref_bssnrhs(double **out, const double **in, unsigned int offset,
+                const double *pmin, const double *pmax, const unsigned int *sz,
+                unsigned int bflag, const unsigned int *lambda,
+                const double *lambda_f, double eta_const, double eta_r0,
+                double eta_damping_exp, double ko_sigma)
+{
+    for (int i = 0; i < 4; i++) bssn::BSSN_LAMBDA[i] = lambda[i];
+    bssn::BSSN_LAMBDA_F[0] = lambda_f[0];
+    bssn::BSSN_LAMBDA_F[1] = lambda_f[1];
+    bssn::ETA_CONST = eta_const;
+    bssn::ETA_R0 = eta_r0;
+    bssn::ETA_DAMPING_EXP = eta_damping_exp;
+    bssn::KO_DISS_SIGMA = ko_sigma;
+    const unsigned int off = offset;
+    const unsigned int bf = bflag;
+    bssnrhs(out, in, off, pmin, pmax, sz, bf);
+    return 0;
+}
+
+// Single stencil sweeps (bssn/src/derivs.cpp), selected by a small integer:
+// 0..2 deriv_{x,y,z}, 3..5 deriv_{xx,yy,zz}, 6..8 adv_deriv_{x,y,z},
+// 9..11 ko_deriv_{x,y,z}.  `beta` is only read by the advective sweeps.
+int ref_deriv(int which, double *out, const double *u, double h,
+              const unsigned int *sz, const double *beta, unsigned int bflag)
+{
+    switch (which) {
+    case 0: deriv_x(out, u, h, sz, bflag); break;
+    case 1: deriv_y(out, u, h, sz, bflag); break;
+    case 2: deriv_z(out, u, h, sz, bflag); break;
+    case 3: deriv_xx(out, u, h, sz, bflag); break;
+    case 4: deriv_yy(out, u, h, sz, bflag); break;
+    case 5: deriv_zz(out, u, h, sz, bflag); break;
+    case 6: adv_deriv_x(out, u, h, sz, beta, bflag); break;
+    case 7: adv_deriv_y(out, u, h, sz, beta, bflag); break;
+    case 8: adv_deriv_z(out, u, h, sz, beta, bflag); break;
+    case 9: ko_deriv_x(out, u, h, sz, bflag); break;
+    case 10: ko_deriv_y(out, u, h, sz, bflag); break;
+    case 11: ko_deriv_z(out, u, h, sz, bflag); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+void ref_fake_initial_data(double x, double y, double z, double *u)
+{
+    bssn::fake_initial_data(x, y, z, u);
+}
+
+}

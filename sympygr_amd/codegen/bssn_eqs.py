@@ -1,0 +1,293 @@
+"""Symbolic statement of the BSSN right-hand side (standard puncture gauge,
+constant eta), written as an ordered list of staged assignments.
+
+This is this project's own statement of the equations the reference states in
+GR/bssn.py:57-104 (``bssn_puncture_gauge(eta, isStaged=False)``) with the
+tensor helpers of GR/dendro.py:170-527.  The result is mathematically equal
+to the reference's generated pointwise code
+(bssn/src/bssneqs_eta_const_standard_gauge.cpp); it differs only in how the
+arithmetic is grouped:
+
+* intermediate tensors (inverse metric, Christoffel symbols, Ricci pieces,
+  raised A~) are kept as named stages instead of being substituted into one
+  expression and CSE'd as a whole (GR/dendro.py:541-563);
+* the quadratic Christoffel terms of the conformal Ricci tensor are contracted
+  through a once-raised first-kind Christoffel ``C1u[a][b][l] =
+  sum_m igt[l][m] C1[a][b][m]``, which removes about a third of the flops.
+
+Variable names follow the reference: fields ``alpha chi K Gt0..2 beta0..2
+B0..2 gt0..5 At0..5`` (bssn/include/grDef.h:31, symmetric order
+xx,xy,xz,yy,yz,zz as GR/dendro.py:86-98), first derivatives
+``grad_<d>_<f>``, second derivatives ``grad2_<i>_<j>_<f>`` with i<=j and
+advective derivatives ``agrad_<d>_<f>`` (bssn/scripts/bssnrhs_derivs*.h).
+"""
+from __future__ import annotations
+
+import sympy as sp
+
+# Field order of the unzipped arrays, bssn/include/grDef.h:31.
+FIELDS = [
+    "alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2",
+    "B0", "B1", "B2", "gt0", "gt1", "gt2", "gt3", "gt4", "gt5",
+    "At0", "At1", "At2", "At3", "At4", "At5",
+]
+FIELD_INDEX = {n: i for i, n in enumerate(FIELDS)}
+
+# Output names in the same order (bssn/src/rhs.cpp:114-137).
+RHS_NAMES = [
+    "a_rhs", "chi_rhs", "K_rhs", "Gt_rhs0", "Gt_rhs1", "Gt_rhs2",
+    "b_rhs0", "b_rhs1", "b_rhs2", "B_rhs0", "B_rhs1", "B_rhs2",
+    "gt_rhs00", "gt_rhs01", "gt_rhs02", "gt_rhs11", "gt_rhs12", "gt_rhs22",
+    "At_rhs00", "At_rhs01", "At_rhs02", "At_rhs11", "At_rhs12", "At_rhs22",
+]
+
+# symmetric 3x3 storage slot of (i, j)
+SYM = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 1): 3, (1, 2): 4, (2, 2): 5}
+
+
+def sym_slot(i: int, j: int) -> int:
+    return SYM[(min(i, j), max(i, j))]
+
+
+E3 = (0, 1, 2)
+E33 = [(i, j) for i in E3 for j in E3]
+
+
+class Inputs:
+    """Symbols for everything the pointwise algebra reads at one point."""
+
+    def __init__(self):
+        self.u = {n: sp.Symbol(n) for n in FIELDS}
+        self.params = {
+            "lambda0": sp.Symbol("lambda0"), "lambda1": sp.Symbol("lambda1"),
+            "lambda2": sp.Symbol("lambda2"), "lambda3": sp.Symbol("lambda3"),
+            "lambda_f0": sp.Symbol("lambda_f0"), "lambda_f1": sp.Symbol("lambda_f1"),
+            "eta": sp.Symbol("eta"),
+        }
+
+    # d_i f  (centred, with bflag closures)
+    @staticmethod
+    def d1(f: str, i: int) -> sp.Symbol:
+        return sp.Symbol(f"grad_{i}_{f}")
+
+    # d_i d_j f, symmetric
+    @staticmethod
+    def d2(f: str, i: int, j: int) -> sp.Symbol:
+        a, b = min(i, j), max(i, j)
+        return sp.Symbol(f"grad2_{a}_{b}_{f}")
+
+    # advective d_i f (upwinded by sign(beta^i), or = d1 in centred mode)
+    @staticmethod
+    def ad(f: str, i: int) -> sp.Symbol:
+        return sp.Symbol(f"agrad_{i}_{f}")
+
+
+def _sym3(prefix: str):
+    return [[sp.Symbol(f"{prefix}{sym_slot(i, j)}") for j in E3] for i in E3]
+
+
+class Stages:
+    """Ordered list of named stages; each stage is a list of (symbol, expr)
+    whose expressions only reference inputs and symbols of earlier stages."""
+
+    def __init__(self):
+        self.stages: list[tuple[str, list[tuple[sp.Symbol, sp.Expr]]]] = []
+        self.outputs: list[tuple[str, sp.Expr]] = []
+
+    def stage(self, name: str, pairs):
+        self.stages.append((name, list(pairs)))
+
+
+def bssn_rhs_stages() -> Stages:
+    """BSSN RHS, GR/bssn.py:57-104, as staged assignments."""
+    I = Inputs()
+    u = I.u
+    P = I.params
+    d1, d2, ad = I.d1, I.d2, I.ad
+    S = Stages()
+
+    alpha, chi, K = u["alpha"], u["chi"], u["K"]
+    b = [u["beta0"], u["beta1"], u["beta2"]]
+    bn = ["beta0", "beta1", "beta2"]
+    B = [u["B0"], u["B1"], u["B2"]]
+    Bn = ["B0", "B1", "B2"]
+    Gt = [u["Gt0"], u["Gt1"], u["Gt2"]]
+    Gtn = ["Gt0", "Gt1", "Gt2"]
+    gtn = [[f"gt{sym_slot(i, j)}" for j in E3] for i in E3]
+    Atn = [[f"At{sym_slot(i, j)}" for j in E3] for i in E3]
+    gt = [[u[gtn[i][j]] for j in E3] for i in E3]
+    At = [[u[Atn[i][j]] for j in E3] for i in E3]
+    half = sp.Rational(1, 2)
+
+    # ---- inverse conformal metric: adjugate / det (GR/dendro.py:374-389,
+    # inv('ADJ')).  The determinant is not assumed to be one.
+    cof = [[None] * 3 for _ in E3]
+    cof[0][0] = gt[1][1] * gt[2][2] - gt[1][2] ** 2
+    cof[0][1] = gt[0][2] * gt[1][2] - gt[0][1] * gt[2][2]
+    cof[0][2] = gt[0][1] * gt[1][2] - gt[0][2] * gt[1][1]
+    cof[1][1] = gt[0][0] * gt[2][2] - gt[0][2] ** 2
+    cof[1][2] = gt[0][1] * gt[0][2] - gt[0][0] * gt[1][2]
+    cof[2][2] = gt[0][0] * gt[1][1] - gt[0][1] ** 2
+    detg = gt[0][0] * cof[0][0] + gt[0][1] * cof[0][1] + gt[0][2] * cof[0][2]
+    idet = sp.Symbol("idet")
+    S.stage("idet", [(idet, 1 / detg)])
+    igt = _sym3("igt")
+    S.stage("igt", [(igt[i][j], cof[i][j] * idet) for (i, j) in SYM])
+
+    inv_chi = sp.Symbol("inv_chi")
+    S.stage("inv_chi", [(inv_chi, 1 / chi)])
+
+    # ---- first-kind Christoffel, GR/dendro.py:392-415:
+    # C1[k,i,j] = 1/2 (d_j g_ki + d_i g_kj - d_k g_ij)
+    C1 = [[[None] * 3 for _ in E3] for _ in E3]
+    pairs = []
+    for k in E3:
+        for (i, j) in SYM:
+            s = sp.Symbol(f"C1_{k}_{i}{j}")
+            pairs.append((s, half * (d1(gtn[k][i], j) + d1(gtn[k][j], i) - d1(gtn[i][j], k))))
+            C1[k][i][j] = C1[k][j][i] = s
+    S.stage("C1", pairs)
+
+    # ---- second-kind Christoffel, GR/dendro.py:418-437: C2[i,j,k] = igt[i,l] C1[l,j,k]
+    C2 = [[[None] * 3 for _ in E3] for _ in E3]
+    pairs = []
+    for i in E3:
+        for (j, k) in SYM:
+            s = sp.Symbol(f"C2_{i}_{j}{k}")
+            pairs.append((s, sum(igt[i][l] * C1[l][j][k] for l in E3)))
+            C2[i][j][k] = C2[i][k][j] = s
+    S.stage("C2", pairs)
+
+    # igt^{im} d_m chi and the raised first-kind symbol used by Ricci
+    dchi = [d1("chi", i) for i in E3]
+    igt_dchi = [sp.Symbol(f"igt_dchi{i}") for i in E3]
+    S.stage("igt_dchi", [(igt_dchi[i], sum(igt[i][m] * dchi[m] for m in E3)) for i in E3])
+
+    # ---- physical-metric Christoffel, GR/dendro.py:440-465:
+    # C3[i,j,k] = C2[i,j,k] - 1/(2 chi) (delta_ij d_k chi + delta_ik d_j chi - gt_jk igt^{im} d_m chi)
+    C3 = [[[None] * 3 for _ in E3] for _ in E3]
+    pairs = []
+    for i in E3:
+        for (j, k) in SYM:
+            s = sp.Symbol(f"C3_{i}_{j}{k}")
+            dij = 1 if i == j else 0
+            dik = 1 if i == k else 0
+            pairs.append((s, C2[i][j][k] - half * inv_chi * (dij * dchi[k] + dik * dchi[j] - gt[j][k] * igt_dchi[i])))
+            C3[i][j][k] = C3[i][k][j] = s
+    S.stage("C3", pairs)
+
+    # ---- conformal connection from the metric, GR/dendro.py:493: CalGt^i = igt^{kl} C2[i,k,l]
+    CalGt = [sp.Symbol(f"CalGt{i}") for i in E3]
+    S.stage("CalGt", [(CalGt[i], sum(igt[k][l] * C2[i][k][l] for (k, l) in E33)) for i in E3])
+
+    # C1u[a][b][l] = sum_m igt[l][m] C1[a][b][m]
+    C1u = [[[sp.Symbol(f"C1u_{a}_{bb}_{l}") for l in E3] for bb in E3] for a in E3]
+    S.stage("C1u", [(C1u[a][bb][l], sum(igt[l][m] * C1[a][bb][m] for m in E3))
+                    for a in E3 for bb in E3 for l in E3])
+
+    # ---- conformal Ricci, GR/dendro.py:495-499:
+    # Rt_ij = -1/2 igt^{lm} d_l d_m gt_ij + 1/2 (gt_ki d_j Gt^k + gt_kj d_i Gt^k)
+    #       + 1/2 CalGt^k (C1[i,j,k] + C1[j,i,k])
+    #       + igt^{lm} (C2[k,l,i] C1[j,k,m] + C2[k,l,j] C1[i,k,m] + C2[k,i,m] C1[k,l,j])
+    # where igt^{lm} C1[j,k,m] = C1u[j][k][l] and igt^{lm} C1[k,l,j] = C1u[k][j][m].
+    Rt = [[None] * 3 for _ in E3]
+    pairs = []
+    for (i, j) in SYM:
+        s = sp.Symbol(f"Rt{sym_slot(i, j)}")
+        e = -half * sum(igt[l][m] * d2(gtn[i][j], l, m) for (l, m) in E33)
+        e += half * sum(gt[k][i] * d1(Gtn[k], j) + gt[k][j] * d1(Gtn[k], i) for k in E3)
+        e += half * sum(CalGt[k] * (C1[i][j][k] + C1[j][i][k]) for k in E3)
+        e += sum(C2[k][l][i] * C1u[j][k][l] + C2[k][l][j] * C1u[i][k][l] for k in E3 for l in E3)
+        e += sum(C2[k][i][m] * C1u[k][j][m] for k in E3 for m in E3)
+        pairs.append((s, e))
+        Rt[i][j] = Rt[j][i] = s
+    S.stage("Rt", pairs)
+
+    # ---- conformal-factor part of Ricci, GR/dendro.py:509-518
+    trchi = sp.Symbol("Rphi_tr")
+    S.stage("Rphi_tr", [(trchi, sum(igt[k][l] * (d2("chi", k, l) - sp.Rational(3, 2) * inv_chi * dchi[k] * dchi[l]) for (k, l) in E33)
+                         - sum(CalGt[m] * dchi[m] for m in E3))])
+    Rphi = [[None] * 3 for _ in E3]
+    pairs = []
+    for (i, j) in SYM:
+        s = sp.Symbol(f"Rphi{sym_slot(i, j)}")
+        e = half * inv_chi * (d2("chi", i, j) - sum(C2[k][j][i] * dchi[k] for k in E3)) \
+            - sp.Rational(1, 4) * inv_chi ** 2 * dchi[i] * dchi[j] \
+            + half * inv_chi * gt[i][j] * trchi
+        pairs.append((s, e))
+        Rphi[i][j] = Rphi[j][i] = s
+    S.stage("Rphi", pairs)
+
+    # ---- D_i D_j alpha with the physical metric, GR/dendro.py:177-188
+    DDa = [[None] * 3 for _ in E3]
+    pairs = []
+    for (i, j) in SYM:
+        s = sp.Symbol(f"DDa{sym_slot(i, j)}")
+        pairs.append((s, d2("alpha", i, j) - sum(C3[l][i][j] * d1("alpha", l) for l in E3)))
+        DDa[i][j] = DDa[j][i] = s
+    S.stage("DDa", pairs)
+
+    # ---- raised A~: At^{ij} (GR/dendro.py:206-214) and A_ik igt^kl A_lj
+    AtU = [[sp.Symbol(f"AtU{sym_slot(i, j)}") for j in E3] for i in E3]
+    Amix = [[sp.Symbol(f"Amix_{i}{j}") for j in E3] for i in E3]  # igt^{ik} A_kj
+    S.stage("Amix", [(Amix[i][j], sum(igt[i][k] * At[k][j] for k in E3)) for i in E3 for j in E3])
+    S.stage("AtU", [(AtU[i][j], sum(Amix[i][k] * igt[k][j] for k in E3)) for (i, j) in SYM])
+    AA = [[sp.Symbol(f"AA{sym_slot(i, j)}") for j in E3] for i in E3]
+    S.stage("AA", [(AA[i][j], sum(At[i][k] * Amix[k][j] for k in E3)) for (i, j) in SYM])
+
+    divb = sp.Symbol("div_beta")
+    S.stage("div_beta", [(divb, sum(d1(bn[k], k) for k in E3))])
+
+    # ---- the RHS, GR/bssn.py:67-100
+    out = {}
+    out["a_rhs"] = P["lambda0"] * sum(b[i] * ad("alpha", i) for i in E3) - 2 * alpha * K
+    for i in E3:
+        out[f"b_rhs{i}"] = sp.Rational(3, 4) * (P["lambda_f0"] + P["lambda_f1"] * alpha) * B[i] \
+            + P["lambda1"] * sum(b[j] * ad(bn[i], j) for j in E3)
+    w = -sp.Rational(2, 3)
+    for (i, j) in SYM:
+        lie = sum(b[k] * ad(gtn[i][j], k) + gt[i][k] * d1(bn[k], j) + gt[k][j] * d1(bn[k], i) for k in E3) \
+            + w * gt[i][j] * divb
+        out[f"gt_rhs{i}{j}"] = lie - 2 * alpha * At[i][j]
+    out["chi_rhs"] = sum(b[i] * ad("chi", i) for i in E3) + w * chi * divb + sp.Rational(2, 3) * chi * alpha * K
+
+    # trace-free part of (alpha R - DDa), GR/dendro.py:321-336
+    X = [[None] * 3 for _ in E3]
+    for (i, j) in SYM:
+        X[i][j] = X[j][i] = alpha * (Rt[i][j] + Rphi[i][j]) - DDa[i][j]
+    trX = sp.Symbol("trX")
+    S.stage("trX", [(trX, sum(igt[k][l] * X[k][l] for (k, l) in E33))])
+    for (i, j) in SYM:
+        lie = sum(b[k] * ad(Atn[i][j], k) + At[i][k] * d1(bn[k], j) + At[k][j] * d1(bn[k], i) for k in E3) \
+            + w * At[i][j] * divb
+        out[f"At_rhs{i}{j}"] = lie + chi * (X[i][j] - gt[i][j] * trX * sp.Rational(1, 3)) \
+            + alpha * (K * At[i][j] - 2 * AA[i][j])
+
+    # K: lie - laplacian(alpha) + alpha (K^2/3 + A_ij A^ij); laplacian uses
+    # the inverse physical metric chi igt (GR/dendro.py:259-275)
+    out["K_rhs"] = sum(b[i] * ad("K", i) for i in E3) \
+        - chi * sum(igt[i][j] * DDa[i][j] for (i, j) in E33) \
+        + alpha * (K * K * sp.Rational(1, 3) + sum(At[i][j] * AtU[i][j] for (i, j) in E33))
+
+    Gt_rhs = []
+    for i in E3:
+        e = sum(b[j] * ad(Gtn[i], j) for j in E3)
+        e -= sum(CalGt[j] * d1(bn[i], j) for j in E3)
+        e += sp.Rational(2, 3) * CalGt[i] * divb
+        e += sum(igt[j][k] * d2(bn[i], j, k) + igt[i][j] * d2(bn[k], j, k) * sp.Rational(1, 3) for (j, k) in E33)
+        e -= sum(2 * AtU[i][j] * d1("alpha", j) for j in E3)
+        e += sum(2 * alpha * C2[i][j][k] * AtU[j][k] for (j, k) in E33)
+        e -= sum(alpha * (3 * inv_chi * AtU[i][j] * dchi[j] + sp.Rational(4, 3) * igt[i][j] * d1("K", j)) for j in E3)
+        Gt_rhs.append(e)
+    Gts = [sp.Symbol(f"Gt_rhs_s{i}") for i in E3]
+    S.stage("Gt_rhs", [(Gts[i], Gt_rhs[i]) for i in E3])
+    for i in E3:
+        out[f"Gt_rhs{i}"] = Gts[i]
+        out[f"B_rhs{i}"] = Gts[i] - P["eta"] * B[i] + P["lambda2"] * sum(b[j] * ad(Bn[i], j) for j in E3) \
+            - P["lambda3"] * sum(b[j] * ad(Gtn[i], j) for j in E3)
+
+    name_map = {"gt_rhs00": "gt_rhs00", "gt_rhs01": "gt_rhs01", "gt_rhs02": "gt_rhs02",
+                "gt_rhs11": "gt_rhs11", "gt_rhs12": "gt_rhs12", "gt_rhs22": "gt_rhs22"}
+    for n in RHS_NAMES:
+        S.outputs.append((name_map.get(n, n), out[n]))
+    return S
