@@ -1,0 +1,125 @@
+/* sympygr_amd -- BSSN right-hand side on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libsympygr_amd.so.  Plain pointers and sizes only; field arrays
+ * use the reference's unzipped layout: 24 arrays in bssn::VAR order
+ * (bssn/include/grDef.h:31), each holding every ghost-padded block
+ * back to back, block b starting at its `offset`, point (i,j,k) of a block at
+ * offset + i + nx*(j + ny*k) (bssn/include/derivs.h:8).  Outputs are written
+ * at interior points [3, n-3)^3 only; ghost entries are left untouched
+ * (bssn/src/rhs.cpp:206-369).
+ *
+ * Every entry point returns 0 on success and a negative SGR_E* code on
+ * failure; sgr_last_error() gives the message of the calling thread's last
+ * failure.  Nothing calls exit().
+ */
+#ifndef SYMPYGR_AMD_H
+#define SYMPYGR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGR_NUM_VARS 24
+
+enum {
+    SGR_OK = 0,
+    SGR_E_INVALID = -1,     /* bad argument (null pointer, size, flag) */
+    SGR_E_UNSUPPORTED = -2, /* valid for the reference, not handled here */
+    SGR_E_HIP = -3,         /* HIP runtime error (message has the HIP text) */
+    SGR_E_NOMEM = -4,       /* device or host allocation failed */
+    SGR_E_NONFINITE = -5    /* non-finite input detected by sgr_check_finite */
+};
+
+enum { SGR_ADV_CENTRED = 0, SGR_ADV_UPWIND = 1 };
+enum { SGR_PREC_FP64 = 0, SGR_PREC_MIXED = 1 };
+
+/* Physics and method parameters.  The reference reads these from mutable
+ * bssn:: globals at call time (bssn/src/parameters.cpp:99-109,
+ * bssn/src/rhs.cpp:148-151, 215-219, 330); here they are explicit. */
+typedef struct {
+    uint32_t lambda[4];       /* BSSN_LAMBDA */
+    double lambda_f[2];       /* BSSN_LAMBDA_F */
+    double eta_const;         /* ETA_CONST */
+    double eta_r0;            /* ETA_R0 */
+    double eta_damping_exp;   /* ETA_DAMPING_EXP */
+    double ko_sigma;          /* KO_DISS_SIGMA */
+    int32_t adv_mode;         /* SGR_ADV_CENTRED (as shipped: deriv64adv_* = deriv64_*)
+                                 or SGR_ADV_UPWIND (deriv42adv_*, bssn/src/derivs.cpp:495-906) */
+    int32_t precision;        /* SGR_PREC_FP64, or SGR_PREC_MIXED: FP64 state, stencils,
+                                 KO and boundary terms; FP32 pointwise algebra */
+} sgr_params;
+
+/* Defaults of bssn/src/parameters.cpp:99-109 with adv_mode/precision given. */
+void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision);
+
+/* One ghost-padded block: the per-block arguments of the reference
+ * bssnrhs(..., offset, ptmin, ptmax, sz, bflag) (bssn/include/rhs.h:33-36). */
+typedef struct {
+    uint64_t offset;   /* start of the block in every field array */
+    uint32_t sz[3];    /* padded sizes (interior + 2*3), each >= 13 */
+    uint32_t bflag;    /* bit d: face d is a domain boundary (LEFT RIGHT DOWN UP BACK FRONT) */
+    double pmin[3];    /* coordinates of the first padded point */
+    double pmax[3];    /* coordinates of the last padded point */
+} sgr_block;
+
+typedef struct sgr_plan sgr_plan;
+
+typedef struct {
+    uint64_t blocks;
+    uint64_t tiles;              /* 8^3 work tiles */
+    uint64_t interior_points;
+    uint64_t workspace_bytes;    /* device scratch held by the plan */
+    uint32_t derivative_slots;   /* derivative values staged per point */
+} sgr_plan_info;
+
+/* Replaces the per-block setup of the reference bssnRHS block loop
+ * (bssn/src/rhs.cpp:41-61) and the workspace sizing of its GPU path
+ * (bssn/cuda_gr/src/rhs_cuda.cu:113-135): validates the block table, uploads
+ * it to `device` and allocates the derivative workspace once. */
+int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, const sgr_params *params,
+                    int device);
+int sgr_plan_destroy(sgr_plan *plan);
+int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info);
+
+/* RHS of every block of the plan on `stream`; in/out are DEVICE pointers to
+ * the 24 unzipped arrays.  Asynchronous; allocates nothing, so it can be
+ * captured into a HIP graph.  Replaces the reference's multi-block entry
+ * bssnRHS (bssn/include/rhs.h:31) and its GPU plug-in cuda::computeRHS
+ * (bssn/cuda_gr/include/rhs_cuda.cuh:39). */
+int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
+            hipStream_t stream);
+
+/* One block, DEVICE pointers, synchronous: bssnrhs (bssn/include/rhs.h:33-36)
+ * with the arrays already on the device. */
+int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                  const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                  const sgr_params *params, int device);
+
+/* One block, HOST pointers: stages the block's padded input to the device,
+ * computes, and copies the interior RHS back.  This is the backend of the
+ * C++ drop-in bssnrhs(...) (bssn/include/rhs.h:33-36). */
+int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                     const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                     const sgr_params *params, int device);
+
+/* Returns SGR_E_NONFINITE if any padded input point of any plan block is NaN
+ * or Inf (the reference propagates them silently). */
+int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipStream_t stream);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated)
+ * and returns its full length. */
+size_t sgr_last_error(char *buf, size_t len);
+
+/* Library version string. */
+const char *sgr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYMPYGR_AMD_H */

@@ -1,0 +1,213 @@
+"""sympygr_amd: the BSSN right-hand side of paralab/SymPyGR's Dendro path on
+AMD Instinct MI355X.
+
+The compute lives in libsympygr_amd.so (hand-written HIP for gfx950 behind the
+C ABI of include/sympygr_amd.h).  This module is a thin ctypes binding used by
+the tests and the benchmark; torch only supplies device memory and streams.
+
+    import sympygr_amd as sgr
+    plan = sgr.Plan(blocks, sgr.params(adv="upwind", precision="fp64"))
+    plan.rhs(out, u)          # u, out: torch.float64 cuda tensors (24, total)
+
+``bssnrhs(out, in, offset, pmin, pmax, sz, bflag, params)`` mirrors the
+reference entry point (bssn/include/rhs.h:33-36) on host arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsympygr_amd.so")
+NUM_VARS = 24
+
+VAR_NAMES = ["alpha", "chi", "K", "Gt0", "Gt1", "Gt2", "beta0", "beta1", "beta2", "B0", "B1", "B2",
+             "gt0", "gt1", "gt2", "gt3", "gt4", "gt5", "At0", "At1", "At2", "At3", "At4", "At5"]
+
+ADV = {"centred": 0, "upwind": 1}
+PREC = {"fp64": 0, "mixed": 1}
+ERRORS = {0: "SGR_OK", -1: "SGR_E_INVALID", -2: "SGR_E_UNSUPPORTED", -3: "SGR_E_HIP", -4: "SGR_E_NOMEM",
+          -5: "SGR_E_NONFINITE"}
+
+P_D = ctypes.POINTER(ctypes.c_double)
+
+
+class SgrParams(ctypes.Structure):
+    _fields_ = [
+        ("lambda_", ctypes.c_uint32 * 4),
+        ("lambda_f", ctypes.c_double * 2),
+        ("eta_const", ctypes.c_double),
+        ("eta_r0", ctypes.c_double),
+        ("eta_damping_exp", ctypes.c_double),
+        ("ko_sigma", ctypes.c_double),
+        ("adv_mode", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+class SgrBlock(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_uint64),
+        ("sz", ctypes.c_uint32 * 3),
+        ("bflag", ctypes.c_uint32),
+        ("pmin", ctypes.c_double * 3),
+        ("pmax", ctypes.c_double * 3),
+    ]
+
+
+class SgrPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("blocks", ctypes.c_uint64),
+        ("tiles", ctypes.c_uint64),
+        ("interior_points", ctypes.c_uint64),
+        ("workspace_bytes", ctypes.c_uint64),
+        ("derivative_slots", ctypes.c_uint32),
+    ]
+
+
+class SgrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# exported functions: name -> (restype, argtypes)
+_SIGS = {
+    "sgr_params_default": (None, [ctypes.POINTER(SgrParams), ctypes.c_int32, ctypes.c_int32]),
+    "sgr_plan_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(SgrBlock), ctypes.c_uint32,
+                                       ctypes.POINTER(SgrParams), ctypes.c_int]),
+    "sgr_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "sgr_plan_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SgrPlanInfo)]),
+    "sgr_rhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_rhs_block": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(SgrParams),
+                                     ctypes.c_int]),
+    "sgr_bssnrhs_host": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
+                                        ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(SgrParams),
+                                        ctypes.c_int]),
+    "sgr_check_finite": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+    "sgr_version": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load libsympygr_amd.so (fails loudly when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m sympygr_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().sgr_last_error(buf, 1024)
+    return buf.value.decode()
+
+
+def check(rc):
+    if rc != 0:
+        raise SgrError(rc, last_error())
+    return rc
+
+
+def params(adv="upwind", precision="fp64", ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
+           lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0)) -> SgrParams:
+    p = SgrParams()
+    lib().sgr_params_default(ctypes.byref(p), ADV[adv], PREC[precision])
+    for i in range(4):
+        p.lambda_[i] = lambda_[i]
+    p.lambda_f[0], p.lambda_f[1] = lambda_f
+    p.eta_const, p.eta_r0, p.eta_damping_exp, p.ko_sigma = eta_const, eta_r0, eta_damping_exp, ko_sigma
+    return p
+
+
+def block_array(bl):
+    """ctypes array of SgrBlock from a sympygr_amd.data.BlockList."""
+    arr = (SgrBlock * max(len(bl), 1))()
+    for b in range(len(bl)):
+        arr[b].offset = bl.offsets[b]
+        for d in range(3):
+            arr[b].sz[d] = bl.sz[b][d]
+            arr[b].pmin[d] = bl.pmin[b][d]
+            arr[b].pmax[d] = bl.pmax[b][d]
+        arr[b].bflag = bl.bflag[b]
+    return arr
+
+
+def _field_ptrs(t):
+    """24 device pointers of the rows of a (24, total) tensor."""
+    if t.dim() != 2 or t.shape[0] != NUM_VARS:
+        raise ValueError("expected a (24, total) tensor")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    base = t.data_ptr()
+    stride = t.stride(0) * t.element_size()
+    return (P_D * NUM_VARS)(*[ctypes.cast(base + f * stride, P_D) for f in range(NUM_VARS)])
+
+
+class Plan:
+    """A validated block table on one device plus its workspace."""
+
+    def __init__(self, blocks, prm: SgrParams, device: int = 0):
+        self._h = ctypes.c_void_p()
+        self.blocks = blocks
+        self.params = prm
+        self._barr = block_array(blocks)
+        check(lib().sgr_plan_create(ctypes.byref(self._h), self._barr, len(blocks), ctypes.byref(prm), device))
+        self.device = device
+
+    def info(self) -> SgrPlanInfo:
+        i = SgrPlanInfo()
+        check(lib().sgr_plan_get_info(self._h, ctypes.byref(i)))
+        return i
+
+    def rhs(self, out, u, stream=None):
+        """Asynchronous RHS on `stream` (default: torch's current stream)."""
+        import torch
+        if u.dtype != torch.float64 or out.dtype != torch.float64 or not u.is_cuda or not out.is_cuda:
+            raise ValueError("u and out must be float64 cuda tensors")
+        if u.shape[1] < self.blocks.total or out.shape[1] < self.blocks.total:
+            raise ValueError("field arrays are shorter than the block table")
+        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
+        check(lib().sgr_rhs(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s)))
+
+    def check_finite(self, u, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
+        return lib().sgr_check_finite(self._h, _field_ptrs(u), ctypes.c_void_p(s))
+
+    def close(self):
+        if self._h:
+            lib().sgr_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bssnrhs(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
+    """Mirror of the reference bssnrhs (bssn/include/rhs.h:33-36) on host numpy
+    arrays u, out of shape (24, N): reads the padded block at `offset`, writes
+    the interior RHS into `out`, leaves ghost entries untouched."""
+    import numpy as np
+    for a in (u, out):
+        if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[0] != NUM_VARS:
+            raise ValueError("expected C-contiguous float64 arrays of shape (24, N)")
+    pi = (P_D * NUM_VARS)(*[u[f].ctypes.data_as(P_D) for f in range(NUM_VARS)])
+    po = (P_D * NUM_VARS)(*[out[f].ctypes.data_as(P_D) for f in range(NUM_VARS)])
+    check(lib().sgr_bssnrhs_host(po, pi, offset, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+                                 (ctypes.c_uint32 * 3)(*sz), bflag, ctypes.byref(prm), device))
+    return out

@@ -102,7 +102,7 @@ class Emitted:
         self.ntemps = ntemps
 
 
-def emit_stages(S, tmp_prefix="t_", indent="    ") -> Emitted:
+def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=False) -> Emitted:
     names = []
     exprs = []
     for _, pairs in S.stages:
@@ -155,13 +155,18 @@ def emit_stages(S, tmp_prefix="t_", indent="    ") -> Emitted:
         lines.append(f"{indent}const R {sym.name} = {printer.doprint(expr)};")
         emitted.add(sym)
 
-    for s in names:
-        emit(s)
-    for n, e in zip(out_names, out_reduced):
+    if not output_order:
+        for s in names:
+            emit(s)
+    order = output_order or list(range(len(out_names)))
+    for oi in order:
+        n, e = out_names[oi], out_reduced[oi]
         for fs in sorted(e.free_symbols, key=lambda x: x.name):
             if fs in defs:
                 emit(fs)
         load_inputs(e)
         flops += count_flops(e)
-        lines.append(f"{indent}BSSN_OUT({out_names.index(n)}, {printer.doprint(e)});")
+        lines.append(f"{indent}BSSN_OUT({oi}, {printer.doprint(e)});")
+        if barrier:
+            lines.append(f"{indent}BSSN_BARRIER();")
     return Emitted("\n".join(lines) + "\n", flops, inputs, len(emitted))

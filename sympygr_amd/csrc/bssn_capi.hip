@@ -1,0 +1,413 @@
+// C ABI of libsympygr_amd.so: plans (validated, uploaded block tables plus a
+// workspace allocated once), the launch entry points and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sympygr_amd.h"
+#include "bssn_kernels.h"
+
+using namespace sgr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(SGR_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, what)                                                                                   \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess) return hip_fail(e_, what);                                                      \
+    } while (0)
+
+// Stencil constants with the reference's expressions (bssn/src/derivs.cpp:
+// 2058-2061, 2446-2449, 499-502, 919-923); h as bssn/src/rhs.cpp:144-146.
+AxisConst axis_const(double pmin, double pmax, uint32_t n)
+{
+    AxisConst a;
+    a.h = (pmax - pmin) / (n - 1);
+    const double idx = 1.0 / a.h;
+    a.by12 = idx / 12.0;
+    a.by60 = idx / 60.0;
+    const double idx_sqrd = 1.0 / (a.h * a.h);
+    a.sq180 = idx_sqrd / 180.0;
+    a.sq12 = idx_sqrd / 12.0;
+    a.by2 = 0.50 * idx;
+    a.ko_pre = -1.0 / 64.0 / a.h;
+    a.smr3 = 59.0 / 48.0 * 64 * a.h;
+    a.smr2 = 43.0 / 48.0 * 64 * a.h;
+    a.smr1 = 49.0 / 48.0 * 64 * a.h;
+    return a;
+}
+
+int validate_params(const sgr_params *p)
+{
+    if (!p) return fail(SGR_E_INVALID, "params is null");
+    if (p->adv_mode != SGR_ADV_CENTRED && p->adv_mode != SGR_ADV_UPWIND)
+        return fail(SGR_E_INVALID, "adv_mode must be SGR_ADV_CENTRED or SGR_ADV_UPWIND");
+    if (p->precision != SGR_PREC_FP64 && p->precision != SGR_PREC_MIXED)
+        return fail(SGR_E_INVALID, "precision must be SGR_PREC_FP64 or SGR_PREC_MIXED");
+    return SGR_OK;
+}
+
+int validate_block(const sgr_block &b, uint32_t idx)
+{
+    char msg[256];
+    for (int d = 0; d < 3; d++) {
+        if (b.sz[d] < 13) {
+            snprintf(msg, sizeof msg, "block %u: sz[%d]=%u < 13 (stencils need 7 interior points)", idx, d, b.sz[d]);
+            return fail(SGR_E_INVALID, msg);
+        }
+        if ((b.sz[d] - 6) % T == 1) {
+            snprintf(msg, sizeof msg,
+                     "block %u: interior size %u = 1 mod %d is not supported by the 8^3 tiling", idx, b.sz[d] - 6, T);
+            return fail(SGR_E_UNSUPPORTED, msg);
+        }
+        if (!(b.pmax[d] > b.pmin[d]) || !std::isfinite(b.pmin[d]) || !std::isfinite(b.pmax[d])) {
+            snprintf(msg, sizeof msg, "block %u: pmax[%d] must be finite and > pmin[%d]", idx, d, d);
+            return fail(SGR_E_INVALID, msg);
+        }
+    }
+    if (b.bflag > 63) {
+        snprintf(msg, sizeof msg, "block %u: bflag %u has bits above the six faces", idx, b.bflag);
+        return fail(SGR_E_INVALID, msg);
+    }
+    return SGR_OK;
+}
+
+Params device_params(const sgr_params &p)
+{
+    Params q;
+    for (int i = 0; i < 4; i++) q.lambda[i] = (double)p.lambda[i];
+    q.lambda_f[0] = p.lambda_f[0];
+    q.lambda_f[1] = p.lambda_f[1];
+    q.eta_const = p.eta_const;
+    q.eta_r0 = p.eta_r0;
+    q.eta_damping_exp = p.eta_damping_exp;
+    q.ko_sigma = p.ko_sigma;
+    return q;
+}
+
+__global__ void k_check_finite(FieldPtrs in, const BlockDesc *blocks, uint32_t nblocks, int *flag)
+{
+    for (uint32_t b = blockIdx.y; b < nblocks; b += gridDim.y) {
+        const BlockDesc &B = blocks[b];
+        const uint64_t n = (uint64_t)B.n[0] * B.n[1] * B.n[2];
+        for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+            bool bad = false;
+            for (int f = 0; f < NF; f++) bad |= !isfinite(in.p[f][B.offset + e]);
+            if (bad) atomicOr(flag, 1);
+        }
+    }
+}
+
+}  // namespace
+
+struct sgr_plan {
+    int device = 0;
+    sgr_params params{};
+    std::vector<BlockDesc> hblocks;
+    BlockDesc *d_blocks = nullptr;
+    uint32_t *d_tile_block = nullptr;
+    uint32_t ntiles = 0;
+    void *d_ws = nullptr;
+    double *d_wsK = nullptr;
+    int *d_flag = nullptr;
+    uint64_t np = 0;
+    uint64_t ws_bytes = 0;
+    uint64_t interior = 0;
+    uint32_t nslots = 0;
+};
+
+static void plan_free(sgr_plan *p)
+{
+    if (!p) return;
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(p->device);
+    hipFree(p->d_blocks);
+    hipFree(p->d_tile_block);
+    hipFree(p->d_ws);
+    hipFree(p->d_wsK);
+    hipFree(p->d_flag);
+    hipSetDevice(cur);
+    delete p;
+}
+
+extern "C" {
+
+void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision)
+{
+    if (!p) return;
+    for (int i = 0; i < 4; i++) p->lambda[i] = 1;
+    p->lambda_f[0] = 1.0;
+    p->lambda_f[1] = 0.0;
+    p->eta_const = 2.0;
+    p->eta_r0 = 50.0;
+    p->eta_damping_exp = 1.0;
+    p->ko_sigma = 0.01;
+    p->adv_mode = adv_mode;
+    p->precision = precision;
+}
+
+int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, const sgr_params *params, int device)
+{
+    if (!plan) return fail(SGR_E_INVALID, "plan is null");
+    *plan = nullptr;
+    if (nblocks > 0 && !blocks) return fail(SGR_E_INVALID, "blocks is null");
+    int rc = validate_params(params);
+    if (rc) return rc;
+    sgr_plan *p = new (std::nothrow) sgr_plan;
+    if (!p) return fail(SGR_E_NOMEM, "host allocation failed");
+    p->device = device;
+    p->params = *params;
+    p->nslots = params->adv_mode == SGR_ADV_UPWIND ? NSLOT_UPWIND : NSLOT_CENTRED;
+    std::vector<uint32_t> tile_block;
+    uint64_t ws_base = 0;
+    for (uint32_t b = 0; b < nblocks; b++) {
+        rc = validate_block(blocks[b], b);
+        if (rc) {
+            delete p;
+            return rc;
+        }
+        BlockDesc d{};
+        d.offset = blocks[b].offset;
+        d.ws_base = ws_base;
+        d.bflag = blocks[b].bflag;
+        d.tile_base = (uint32_t)tile_block.size();
+        uint64_t nt = 1;
+        for (int k = 0; k < 3; k++) {
+            d.n[k] = (int32_t)blocks[b].sz[k];
+            d.ntile[k] = (int32_t)((blocks[b].sz[k] - 6 + T - 1) / T);
+            d.pmin[k] = blocks[b].pmin[k];
+            d.ax[k] = axis_const(blocks[b].pmin[k], blocks[b].pmax[k], blocks[b].sz[k]);
+            nt *= (uint64_t)d.ntile[k];
+        }
+        if (tile_block.size() + nt > 0xFFFFFFF0u) {
+            delete p;
+            return fail(SGR_E_INVALID, "too many tiles");
+        }
+        for (uint64_t t = 0; t < nt; t++) tile_block.push_back(b);
+        ws_base += nt * TILE_PTS;
+        p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
+        p->hblocks.push_back(d);
+    }
+    int ndev = 0;
+    hipError_t derr = hipGetDeviceCount(&ndev);
+    if (derr != hipSuccess || device < 0 || device >= ndev) {
+        delete p;
+        return derr != hipSuccess ? hip_fail(derr, "hipGetDeviceCount") : fail(SGR_E_INVALID, "device index out of range");
+    }
+    p->ntiles = (uint32_t)tile_block.size();
+    p->np = ws_base;
+    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
+    p->ws_bytes = p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
+
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipError_t e = hipSetDevice(device);
+    auto bail = [&](hipError_t err, const char *what) {
+        hipSetDevice(cur);
+        plan_free(p);
+        return err == hipErrorOutOfMemory ? fail(SGR_E_NOMEM, std::string(what) + ": out of device memory")
+                                          : hip_fail(err, what);
+    };
+    if (e != hipSuccess) return bail(e, "hipSetDevice");
+    if (nblocks) {
+        if ((e = hipMalloc(&p->d_blocks, nblocks * sizeof(BlockDesc))) != hipSuccess) return bail(e, "hipMalloc(blocks)");
+        if ((e = hipMemcpy(p->d_blocks, p->hblocks.data(), nblocks * sizeof(BlockDesc), hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy(blocks)");
+        if ((e = hipMalloc(&p->d_tile_block, tile_block.size() * sizeof(uint32_t))) != hipSuccess)
+            return bail(e, "hipMalloc(tiles)");
+        if ((e = hipMemcpy(p->d_tile_block, tile_block.data(), tile_block.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy(tiles)");
+        if ((e = hipMalloc(&p->d_ws, p->np * (uint64_t)p->nslots * wsz)) != hipSuccess) return bail(e, "hipMalloc(workspace)");
+        if ((e = hipMalloc(&p->d_wsK, p->np * (uint64_t)NF * sizeof(double))) != hipSuccess)
+            return bail(e, "hipMalloc(ko workspace)");
+    }
+    if ((e = hipMalloc(&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
+    hipSetDevice(cur);
+    *plan = p;
+    return SGR_OK;
+}
+
+int sgr_plan_destroy(sgr_plan *plan)
+{
+    plan_free(plan);
+    return SGR_OK;
+}
+
+int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
+{
+    if (!plan || !info) return fail(SGR_E_INVALID, "null argument");
+    info->blocks = plan->hblocks.size();
+    info->tiles = plan->ntiles;
+    info->interior_points = plan->interior;
+    info->workspace_bytes = plan->ws_bytes;
+    info->derivative_slots = plan->nslots;
+    return SGR_OK;
+}
+
+int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], hipStream_t stream)
+{
+    if (!plan || !out || !in) return fail(SGR_E_INVALID, "null argument");
+    LaunchArgs a{};
+    for (int f = 0; f < NF; f++) {
+        if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
+        a.in.p[f] = in[f];
+        a.out.p[f] = out[f];
+    }
+    a.blocks = plan->d_blocks;
+    a.tile_block = plan->d_tile_block;
+    a.ntiles = plan->ntiles;
+    a.ws = plan->d_ws;
+    a.wsK = plan->d_wsK;
+    a.np = plan->np;
+    a.params = device_params(plan->params);
+    a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
+    a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    int cur = 0;
+    hipGetDevice(&cur);
+    if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
+    hipError_t e = launch_rhs(a, stream);
+    if (cur != plan->device) hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return SGR_OK;
+}
+
+int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipStream_t stream)
+{
+    if (!plan || !in) return fail(SGR_E_INVALID, "null argument");
+    FieldPtrs fp;
+    for (int f = 0; f < NF; f++) {
+        if (!in[f]) return fail(SGR_E_INVALID, "null field pointer");
+        fp.p[f] = in[f];
+    }
+    if (plan->hblocks.empty()) return SGR_OK;
+    int cur = 0;
+    hipGetDevice(&cur);
+    HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
+    int h = 0;
+    hipError_t e = hipMemsetAsync(plan->d_flag, 0, sizeof(int), stream);
+    if (e == hipSuccess) {
+        const uint32_t nb = (uint32_t)plan->hblocks.size();
+        hipLaunchKernelGGL(k_check_finite, dim3(64, nb < 65535 ? nb : 65535), dim3(256), 0, stream, fp, plan->d_blocks,
+                           nb, plan->d_flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, plan->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "check_finite");
+    if (h) return fail(SGR_E_NONFINITE, "non-finite value in the padded input of at least one block");
+    return SGR_OK;
+}
+
+int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                  const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                  const sgr_params *params, int device)
+{
+    if (!pmin || !pmax || !sz) return fail(SGR_E_INVALID, "null argument");
+    sgr_block b;
+    b.offset = offset;
+    b.bflag = bflag;
+    for (int d = 0; d < 3; d++) {
+        b.sz[d] = sz[d];
+        b.pmin[d] = pmin[d];
+        b.pmax[d] = pmax[d];
+    }
+    sgr_plan *plan = nullptr;
+    int rc = sgr_plan_create(&plan, &b, 1, params, device);
+    if (rc) return rc;
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(device);
+    rc = sgr_rhs(plan, out, in, nullptr);
+    if (rc == SGR_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
+    }
+    hipSetDevice(cur);
+    sgr_plan_destroy(plan);
+    return rc;
+}
+
+int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                     const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                     const sgr_params *params, int device)
+{
+    if (!out || !in || !pmin || !pmax || !sz) return fail(SGR_E_INVALID, "null argument");
+    for (int f = 0; f < NF; f++)
+        if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
+    const uint64_t n = (uint64_t)sz[0] * sz[1] * sz[2];
+    int cur = 0;
+    hipGetDevice(&cur);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    double *d_in = nullptr, *d_out = nullptr;
+    std::vector<double> h_out;
+    int rc = SGR_OK;
+    hipError_t e = hipMalloc(&d_in, n * NF * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_out, n * NF * sizeof(double));
+    for (int f = 0; f < NF && e == hipSuccess; f++)
+        e = hipMemcpy(d_in + f * n, in[f] + offset, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "staging input");
+    } else {
+        double *o[NF];
+        const double *i[NF];
+        for (int f = 0; f < NF; f++) {
+            o[f] = d_out + f * n;
+            i[f] = d_in + f * n;
+        }
+        rc = sgr_rhs_block(o, i, 0, pmin, pmax, sz, bflag, params, device);
+        if (rc == SGR_OK) {
+            h_out.resize(n * NF);
+            e = hipMemcpy(h_out.data(), d_out, n * NF * sizeof(double), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "copying RHS back");
+        }
+    }
+    hipFree(d_in);
+    hipFree(d_out);
+    hipSetDevice(cur);
+    if (rc) return rc;
+    // only interior points are written, as the reference does
+    const uint32_t nx = sz[0], ny = sz[1], nz = sz[2];
+    for (int f = 0; f < NF; f++)
+        for (uint32_t k = 3; k < nz - 3; k++)
+            for (uint32_t j = 3; j < ny - 3; j++) {
+                const uint64_t row = (uint64_t)nx * (j + (uint64_t)ny * k);
+                std::memcpy(out[f] + offset + row + 3, h_out.data() + f * n + row + 3, (nx - 6) * sizeof(double));
+            }
+    return SGR_OK;
+}
+
+size_t sgr_last_error(char *buf, size_t len)
+{
+    if (buf && len) {
+        const size_t m = g_err.size() < len - 1 ? g_err.size() : len - 1;
+        std::memcpy(buf, g_err.data(), m);
+        buf[m] = 0;
+    }
+    return g_err.size();
+}
+
+const char *sgr_version(void) { return "sympygr_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Shared host/device definitions for the BSSN RHS kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgr {
+
+constexpr int NF = 24;     // evolved fields, order of bssn/include/grDef.h:31
+constexpr int GW = 3;      // ghost width (bssn/include/block.h:21)
+constexpr int T = 8;       // tile edge (interior points)
+constexpr int TH = T + 2 * GW;  // tile edge with halo
+constexpr int TILE_PTS = T * T * T;
+
+// Fields with second derivatives (bssn/scripts/bssnrhs_derivs.h):
+// alpha, beta0..2, chi, gt0..5.  D2Q[f] = position in that list or -1.
+constexpr int ND2F = 11;
+__host__ __device__ constexpr int d2q(int f)
+{
+    return f == 0 ? 0 : f == 6 ? 1 : f == 7 ? 2 : f == 8 ? 3 : f == 1 ? 4 : (f >= 12 && f <= 17) ? 5 + (f - 12) : -1;
+}
+__host__ __device__ constexpr int d2field(int q)
+{
+    return q == 0 ? 0 : q == 1 ? 6 : q == 2 ? 7 : q == 3 ? 8 : q == 4 ? 1 : 12 + (q - 5);
+}
+
+// Derivative workspace slots, one plane of NP points each:
+//   [0,72)     first derivative of field f along d: f*3+d
+//   [72,138)   second derivatives, 72 + q*6 + s (s: xx xy xz yy yz zz)
+//   [138,210)  advective derivatives (upwind mode only), 138 + f*3+d
+constexpr int SLOT_D1 = 0;
+constexpr int SLOT_D2 = 72;
+constexpr int SLOT_AD = 138;
+constexpr int NSLOT_CENTRED = 138;
+constexpr int NSLOT_UPWIND = 210;
+
+__host__ __device__ constexpr int slot_d1(int f, int d) { return SLOT_D1 + f * 3 + d; }
+__host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + d2q(f) * 6 + s; }
+__host__ __device__ constexpr int slot_ad(int f, int d) { return SLOT_AD + f * 3 + d; }
+
+// Sommerfeld falloff / asymptotic value per field (bssn/src/rhs.cpp:261-320).
+__host__ __device__ constexpr double bc_falloff(int f)
+{
+    return (f >= 3 && f <= 5) || f >= 18 ? 2.0 : 1.0;
+}
+__host__ __device__ constexpr double bc_asym(int f)
+{
+    return (f == 0 || f == 1 || f == 12 || f == 15 || f == 17) ? 1.0 : 0.0;
+}
+
+// Per-axis stencil constants, computed on the host with the reference's
+// expressions (bssn/src/derivs.cpp) so the device sees identical doubles.
+struct AxisConst {
+    double h;        // (pmax - pmin)/(n - 1)            rhs.cpp:144-146
+    double by60;     // (1/h)/60                         derivs.cpp:2061
+    double by12;     // (1/h)/12                         derivs.cpp:2060
+    double sq180;    // (1/(h h))/180                    derivs.cpp:2448
+    double sq12;     // (1/(h h))/12                     derivs.cpp:2449
+    double by2;      // 0.5 (1/h)                        derivs.cpp:501
+    double ko_pre;   // -1/64/h                          derivs.cpp:919
+    double smr3, smr2, smr1;  // 59/48*64*h, 43/48*64*h, 49/48*64*h   derivs.cpp:921-923
+};
+
+// One padded block as the kernels see it.
+struct BlockDesc {
+    uint64_t offset;     // start of the block in every field array
+    uint64_t ws_base;    // first workspace point of the block (tile-major)
+    int32_t n[3];        // padded sizes
+    int32_t ntile[3];    // tiles per axis = ceil((n-6)/T)
+    uint32_t bflag;
+    uint32_t tile_base;  // first global tile id of this block
+    double pmin[3];
+    AxisConst ax[3];
+};
+
+// Physics parameters (bssn/src/parameters.cpp:99-109).
+struct Params {
+    double lambda[4];
+    double lambda_f[2];
+    double eta_const, eta_r0, eta_damping_exp;
+    double ko_sigma;
+};
+
+struct FieldPtrs {
+    const double *p[NF];
+};
+struct OutPtrs {
+    double *p[NF];
+};
+
+}  // namespace sgr

@@ -1,0 +1,24 @@
+// Kernel launch interface used by the C-ABI layer.
+#pragma once
+
+#include "bssn_common.h"
+
+namespace sgr {
+
+struct LaunchArgs {
+    FieldPtrs in;
+    OutPtrs out;
+    const BlockDesc *blocks;     // device
+    const uint32_t *tile_block;  // device, block id of every tile
+    uint32_t ntiles;
+    void *ws;                    // derivative workspace, nslot x np (double or float)
+    double *wsK;                 // KO sums, 24 x np
+    uint64_t np;                 // workspace points (ntiles * 512)
+    Params params;
+    bool upwind;
+    bool mixed;
+};
+
+hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);
+
+}  // namespace sgr

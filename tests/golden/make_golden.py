@@ -1,0 +1,94 @@
+"""Write the golden fixtures from the compiled reference (oracle/_ref).
+
+Run in a container that has the reference sources and a built oracle/_ref:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Outputs (all numbers, nothing copied from the reference):
+  rhs_<case>.npy       interior RHS, shape (24, n-6, n-6, n-6), float64
+  stencils.json        SHA-256 of every single-sweep output (bit-exact checks)
+  manifest.json        input recipes, parameters, flags, input hashes
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle as po  # noqa: E402
+from sympygr_amd import data  # noqa: E402
+
+C0 = dict(seed=1, n=22, amp=1e-3, pmin=[-1.05] * 3, pmax=[1.05] * 3)
+FAKE = dict(n=13, pmin=[0.1, 0.2, 0.3], pmax=[1.3, 1.5, 1.4])
+PARAMS = dict(ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0, lambda_=[1, 1, 1, 1], lambda_f=[1.0, 0.0])
+REF_FLAGS = "g++ -std=c++11 -O2 -ffp-contract=off -fopenmp (oracle/Makefile)"
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def interior(a, n):
+    return np.ascontiguousarray(a.reshape(24, n, n, n)[:, 3:-3, 3:-3, 3:-3])
+
+
+def c0_input():
+    return data.perturbed_flat_block(seed=C0["seed"], n=C0["n"], amp=C0["amp"], pmin=tuple(C0["pmin"]),
+                                     pmax=tuple(C0["pmax"]))
+
+
+def fake_input():
+    n = FAKE["n"]
+    return po.ref_fake_initial_data("u", (n, n, n), FAKE["pmin"], FAKE["pmax"])
+
+
+def main():
+    manifest = {"reference_build": REF_FLAGS, "params": PARAMS, "cases": {}}
+    u0 = c0_input()
+    manifest["c0_input"] = dict(C0, recipe="sympygr_amd.data.perturbed_flat_block", sha256=sha(u0))
+    uf = fake_input()
+    np.save(os.path.join(HERE, "fake_input_13.npy"), uf)
+    manifest["fake_input"] = dict(FAKE, recipe="reference bssn::fake_initial_data (bssn/src/grUtils.cpp:15-57)",
+                                  file="fake_input_13.npy", sha256=sha(uf))
+    cases = [
+        ("c0_upwind_b0", u0, C0, "u", 0), ("c0_upwind_b63", u0, C0, "u", 63),
+        ("c0_centred_b63", u0, C0, "c", 63),
+        ("fake_upwind_b0", uf, FAKE, "u", 0), ("fake_upwind_b63", uf, FAKE, "u", 63),
+        ("fake_centred_b0", uf, FAKE, "c", 0), ("fake_centred_b21", uf, FAKE, "c", 21),
+        ("fake_upwind_b42", uf, FAKE, "u", 42),
+    ]
+    for name, u, geo, var, bflag in cases:
+        n = geo["n"]
+        P = po.make_params(upwind=(var == "u"), **PARAMS)
+        out = po.ref_rhs_block(var, u, (n, n, n), geo["pmin"], geo["pmax"], bflag, P)
+        out = interior(out, n)
+        np.save(os.path.join(HERE, f"rhs_{name}.npy"), out)
+        manifest["cases"][name] = dict(input="c0" if geo is C0 else "fake", variant=var, bflag=bflag, n=n,
+                                       file=f"rhs_{name}.npy", sha256=sha(out))
+    # single-sweep hashes on the fake input's gt1 (field 13) with beta from the
+    # same input, 13^3 block, h = 0.1
+    st = {}
+    n = FAKE["n"]
+    for var in ("u", "c"):
+        for bflag in (0, 63, 21, 42):
+            for which in range(12):
+                beta = uf[6 + which % 3] if which // 3 == 2 else None
+                out = po.ref_deriv(var, which, uf[13], 0.1, (n, n, n), bflag, beta=beta)
+                st[f"{var}_{bflag}_{which}"] = sha(interior(np.stack([out] * 24), n)[0])
+    with open(os.path.join(HERE, "stencils.json"), "w") as f:
+        json.dump(st, f, indent=1, sort_keys=True)
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+    print("wrote", len(cases), "rhs fixtures and", len(st), "stencil hashes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+"""GPU parity: the HIP path through the C ABI against the golden fixtures
+(compiled reference) and the CPU oracle.
+
+Tolerances: plain FP64 1e-12 normwise relative per field (max|d|/max|ref|);
+stencil-only quantities are bit-exact by construction and tested as such in
+test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+from helpers import case_input, case_params, golden, interior, manifest, normwise
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _run_gpu(u, sz, pmin, pmax, bflag, adv, precision="fp64", **pkw):
+    torch = _torch()
+    import sympygr_amd as sgr
+    from sympygr_amd.data import BlockList
+    bl = BlockList()
+    bl.add(sz, pmin, pmax, bflag)
+    prm = sgr.params(adv=adv, precision=precision, **pkw)
+    plan = sgr.Plan(bl, prm)
+    du = torch.from_numpy(np.ascontiguousarray(u)).cuda()
+    out = torch.full_like(du, 7.0)
+    plan.rhs(out, du)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+CASES = ["c0_upwind_b0", "c0_upwind_b63", "c0_centred_b63", "fake_upwind_b0", "fake_upwind_b63",
+         "fake_centred_b0", "fake_centred_b21", "fake_upwind_b42"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matches_reference_fixture(name):
+    u, sz, pmin, pmax, c = case_input(name)
+    p, up = case_params(c)
+    print(f"case {name} sz={sz} bflag={c['bflag']}", flush=True)
+    out = _run_gpu(u, sz, pmin, pmax, c["bflag"], "upwind" if up else "centred", ko_sigma=p["ko_sigma"])
+    err = normwise(interior(out, sz), golden(name))
+    assert err.max() <= FP64_TOL, (name, err)
+
+
+def test_ghost_points_untouched():
+    u, sz, pmin, pmax, c = case_input("c0_upwind_b63")
+    out = _run_gpu(u, sz, pmin, pmax, 63, "upwind").reshape(24, sz[2], sz[1], sz[0])
+    mask = np.ones(out.shape[1:], bool)
+    mask[3:-3, 3:-3, 3:-3] = False
+    assert np.all(out[:, mask] == 7.0)
+
+
+@pytest.mark.parametrize("bflag", [0, 1, 2, 12, 48, 63, 37])
+def test_fp64_matches_oracle_any_bflag(bflag):
+    import pyoracle as po
+    u, sz, pmin, pmax, c = case_input("fake_upwind_b0")
+    for adv in ("upwind", "centred"):
+        print(f"bflag {bflag} adv {adv}", flush=True)
+        out = _run_gpu(u, sz, pmin, pmax, bflag, adv)
+        ref = po.rhs_block(u, sz, pmin, pmax, bflag, po.make_params(upwind=adv == "upwind"))
+        err = normwise(interior(out, sz), interior(ref, sz))
+        assert err.max() <= FP64_TOL, (bflag, adv, err)
+
+
+def test_uniform_tiling_blocks_match_oracle():
+    """The 8^3 tiling of [-8,8]^3 (Brill-Lindquist, 32^3 interior blocks): whole
+    batch on the GPU, oracle on a corner, an edge, a face and an interior block."""
+    torch = _torch()
+    import pyoracle as po
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    bl = data.uniform_tiling(nb=8, n=32)
+    u = data.fill_blocks(bl, backend="torch", device="cuda")
+    prm = sgr.params(adv="upwind", precision="fp64")
+    plan = sgr.Plan(bl, prm)
+    out = torch.zeros_like(u)
+    plan.rhs(out, u)
+    torch.cuda.synchronize()
+    pick = [0, 3, 8 * 8 * 3 + 8 * 3 + 0, 8 * 8 * 4 + 8 * 4 + 4]  # corner, edge, face, interior
+    P = po.make_params(upwind=True)
+    for b in pick:
+        o, s = bl.offsets[b], bl.sz[b]
+        n = s[0] * s[1] * s[2]
+        ub = u[:, o:o + n].cpu().numpy()
+        ref = po.rhs_block(ub, s, bl.pmin[b], bl.pmax[b], bl.bflag[b], P)
+        got = out[:, o:o + n].cpu().numpy()
+        err = normwise(interior(got, s), interior(ref, s))
+        print(f"block {b} bflag {bl.bflag[b]} max err {err.max():.2e}", flush=True)
+        assert err.max() <= FP64_TOL, (b, err)
+
+
+def test_mixed_precision_error_bound():
+    """FP32 algebra: per-field normwise error vs the FP64 oracle stays below 1e-6
+    on the perturbed-flat block and the analytic test data."""
+    import pyoracle as po
+    for name in ("c0_upwind_b0", "fake_upwind_b63"):
+        u, sz, pmin, pmax, c = case_input(name)
+        out = _run_gpu(u, sz, pmin, pmax, c["bflag"], "upwind", precision="mixed")
+        ref = po.rhs_block(u, sz, pmin, pmax, c["bflag"], po.make_params(upwind=True))
+        err = normwise(interior(out, sz), interior(ref, sz))
+        print(name, "mixed max normwise err", f"{err.max():.2e}", flush=True)
+        assert err.max() <= 1e-6, (name, err)
+
+
+def test_host_dropin_matches_device_path():
+    """sgr_bssnrhs_host (host arrays, block at a non-zero offset) equals the
+    device entry point and leaves everything outside the interior untouched."""
+    import sympygr_amd as sgr
+    u, sz, pmin, pmax, c = case_input("c0_upwind_b63")
+    n = u.shape[1]
+    big = np.zeros((24, n + 1000))
+    big[:, 1000:] = u
+    out = np.full_like(big, 3.0)
+    sgr.bssnrhs(out, big, 1000, pmin, pmax, sz, 63, sgr.params(adv="upwind"))
+    dev = _run_gpu(u, sz, pmin, pmax, 63, "upwind")
+    got = out[:, 1000:]
+    assert np.all(out[:, :1000] == 3.0)
+    np.testing.assert_array_equal(interior(got, sz), interior(dev, sz))
