@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""BSSN RHS throughput on one MI355X (Mpoints/s = interior points per second
+for one RHS evaluation over the whole block set).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+
+Workload: 8x8x8 blocks of 32^3 interior points + 3 ghost layers (38^3 padded)
+tiling [-8,8]^3 with Brill-Lindquist two-puncture data (masses 0.5 at
+(+-2,0,0), ghosts filled analytically), outer-face blocks carrying bflag,
+6th-order stencils, shift-upwinded advection, KO sigma 0.1.  A step is one
+RHS evaluation of all 512 blocks with the inputs resident in HBM.
+
+`value` is the plain FP64 rate.  The same line carries the mixed-precision
+rate (FP32 pointwise algebra) with its measured error against FP64, the
+roofline of the RHS pass from HIP events recorded around each kernel inside
+the timed loop, and the CPU oracle timed on a bounded sample of the same
+blocks on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BSSN RHS Mpoints/s: plain FP64 vs batched mixed-prec at stated err; % roofline"
+
+# Per interior point of an n^3 block (compulsory, fused single pass): read the
+# 24 FP64 fields over the padded block once, write the 24 FP64 RHS values.
+def alg_bytes_per_point(n: int) -> float:
+    return 192.0 * ((n + 6) / n) ** 3 + 192.0
+
+
+# Algorithmic FP64 flops per interior point, upwind mode: stencils
+# 2*(45*6 + 72*5 + 33*7 + 33*6 + 72*7) = 3126, KO add 24*5 = 120, pointwise
+# algebra 3917 (operation count of the reference's generated expression).
+FLOPS_UPWIND = 7163
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+class HipEvents:
+    """hipEvent_t handles through libamdhip64 (events are recorded by the
+    library around each kernel launch on the launch stream)."""
+
+    def __init__(self):
+        import torch  # noqa: F401  (loads the HIP runtime)
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.h = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = ctypes.c_void_p()
+            if self.h.hipEventCreate(ctypes.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            evs.append(e.value)
+        return evs
+
+    def elapsed(self, a, b):
+        ms = ctypes.c_float()
+        if self.h.hipEventElapsedTime(ctypes.byref(ms), ctypes.c_void_p(a), ctypes.c_void_p(b)) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value
+
+    def destroy(self, evs):
+        for e in evs:
+            self.h.hipEventDestroy(ctypes.c_void_p(e))
+
+
+def timed_steps(plan, out, u, steps, warmup, hev):
+    import torch
+    for _ in range(warmup):
+        plan.rhs(out, u)
+    torch.cuda.synchronize()
+    evs = [hev.create(4) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        plan.rhs_events(out, u, evs[s])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kd = sum(hev.elapsed(e[0], e[1]) for e in evs) / steps
+    kp = sum(hev.elapsed(e[2], e[3]) for e in evs) / steps
+    for e in evs:
+        hev.destroy(e)
+    return wall, kd, kp
+
+
+def normwise_err(a, b):
+    """per-field max|a-b| / max|b|"""
+    import torch
+    d = (a - b).abs().amax(dim=1)
+    r = b.abs().amax(dim=1).clamp_min(1e-300)
+    return (d / r)
+
+
+def cpu_baseline(bl, u, target_s=12.0):
+    """The CPU oracle (this project's C++ restatement of the reference
+    bssnrhs, same structure: full-array sweeps, pointwise loop, BCs, KO) on a
+    bounded sample of the same blocks, OpenMP over blocks on this host."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from sympygr_amd.data import BlockList
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    nsample = max(threads, 16)
+    sub = BlockList()
+    for b in range(nsample):
+        sub.add(bl.sz[b], bl.pmin[b], bl.pmax[b], bl.bflag[b])
+    hu = np.ascontiguousarray(u[:, : sub.total].cpu().numpy())
+    out = np.zeros_like(hu)
+    P = po.make_params(upwind=True, ko_sigma=0.1)
+    po.rhs_blocks(hu, sub, P, nthreads=threads, out=out)  # warm-up
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < target_s / 4 or reps < 2:
+        po.rhs_blocks(hu, sub, P, nthreads=threads, out=out)
+        reps += 1
+        t = time.perf_counter() - t0
+    pts = sub.interior_points() * reps
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": pts / t / 1e6, "unit": "Mpoints/s", "cores": threads, "kind": "port",
+            "sample": f"{nsample} of the 512 blocks (first in z-y-x order, faces included), {reps} RHS passes, "
+                      f"{t:.1f} s wall; OpenMP over blocks; CPU: {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the RHS path is single-GPU here); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    sgr.lib()
+    n = 32
+    bl = data.uniform_tiling(nb=8, n=n, stagger=0.5)
+    log(f"filling {len(bl)} blocks ({bl.total} padded points per field)")
+    u = data.fill_blocks(bl, backend="torch", device="cuda")
+    out = torch.zeros_like(u)
+    pts = bl.interior_points()
+    hev = HipEvents()
+
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64", ko_sigma=0.1))
+    log(f"fp64 plan: workspace {plan.info().workspace_bytes / 2**30:.1f} GiB; warmup {args.warmup}, steps {args.steps}")
+    wall, kd, kp = timed_steps(plan, out, u, args.steps, args.warmup, hev)
+    value = pts * args.steps / wall / 1e6
+    log(f"fp64: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_derivs {kd:.3f} ms, k_pointwise {kp:.3f} ms")
+    ref_out = out.clone()
+    del plan
+
+    mixed = None
+    if not args.no_mixed:
+        mplan = sgr.Plan(bl, sgr.params(adv="upwind", precision="mixed", ko_sigma=0.1))
+        mout = torch.zeros_like(u)
+        mwall, mkd, mkp = timed_steps(mplan, mout, u, args.steps, args.warmup, hev)
+        mval = pts * args.steps / mwall / 1e6
+        err = normwise_err(mout, ref_out)
+        rms = ((mout - ref_out).pow(2).mean(dim=1).sqrt() / ref_out.pow(2).mean(dim=1).sqrt().clamp_min(1e-300))
+        log(f"mixed: {mval:.1f} Mpts/s, k_derivs {mkd:.3f} ms, k_pointwise {mkp:.3f} ms, max err {err.max().item():.2e}")
+        mixed = {"value": mval, "ms_per_step": 1e3 * mwall / args.steps, "speedup_vs_fp64": mval / value,
+                 "kernel_ms": {"k_derivs": mkd, "k_pointwise": mkp},
+                 "max_normwise_rel_err_vs_fp64": err.max().item(),
+                 "per_field_normwise_rel_err": [float(f"{x:.3e}") for x in err.tolist()],
+                 "per_field_rms_rel_err": [float(f"{x:.3e}") for x in rms.tolist()]}
+        del mplan, mout
+
+    rhs_ms = kd + kp
+    abytes = alg_bytes_per_point(n) * pts
+    achieved_gbs = abytes / (rhs_ms * 1e-3) / 1e9
+    achieved_tfs = FLOPS_UPWIND * pts / (rhs_ms * 1e-3) / 1e12
+    dominant = "k_pointwise" if kp >= kd else "k_derivs"
+    roofline = {
+        "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+        "kernel": "RHS pass = k_derivs + k_pointwise (event-timed per launch)",
+        "dominant_kernel": dominant, "kernel_ms": {"k_derivs": kd, "k_pointwise": kp},
+        "algorithmic_bytes_per_point": alg_bytes_per_point(n),
+        "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                        "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
+    }
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle on a bounded sample)")
+        cpu = cpu_baseline(bl, u)
+        log(f"cpu: {cpu['value']:.3f} Mpts/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Brill-Lindquist two-puncture data, analytic ghosts",
+        "config": {"workload": "512 blocks (8^3 tiling of [-8,8]^3) of 32^3 interior + 3 ghost (38^3), "
+                               "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
+                   "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
+        "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,30 @@
+/* C++ drop-in for the reference per-block entry point.
+ *
+ * Declares exactly the reference signature of bssnrhs (bssn/include/rhs.h:33-36),
+ * so a Dendro build links this library instead of bssn/src/rhs.cpp.  The
+ * implementation (libsympygr_amd.so) reads the physics parameters from the
+ * same bssn:: globals the reference reads (bssn/src/parameters.cpp:99-109);
+ * the library carries weak default definitions that the application's own
+ * parameters.cpp overrides at link time.  Arrays are HOST memory, as in the
+ * reference; each call stages the block to the GPU and copies the interior
+ * RHS back.  Errors throw std::runtime_error (the reference has no error
+ * channel).
+ */
+#ifndef SYMPYGR_BSSN_DROPIN_H
+#define SYMPYGR_BSSN_DROPIN_H
+
+#include <stdint.h>
+
+void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
+             const double *ptmax, const unsigned int *sz, const unsigned int &bflag);
+
+extern "C" {
+/* Method switches the reference selects with compile-time macros
+ * (BSSN_USE_ADVECTIVE_DERIVS / deriv42adv_*, bssn/include/derivs.h:68-103):
+ * adv_mode SGR_ADV_CENTRED or SGR_ADV_UPWIND, precision SGR_PREC_FP64 or
+ * SGR_PREC_MIXED, and the HIP device used by bssnrhs.  Default: centred
+ * (the shipped 6th-order build), FP64, device 0. */
+void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device);
+}
+
+#endif

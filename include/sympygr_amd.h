@@ -94,6 +94,13 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info);
 int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
             hipStream_t stream);
 
+/* Same as sgr_rhs, and records events[0..1] around the derivative kernel and
+ * events[2..3] around the pointwise kernel on `stream` (hipEventElapsedTime
+ * then gives each kernel's duration).  No reference counterpart: it exists
+ * for measurement. */
+int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
+                   hipStream_t stream, hipEvent_t events[4]);
+
 /* One block, DEVICE pointers, synchronous: bssnrhs (bssn/include/rhs.h:33-36)
  * with the arrays already on the device. */
 int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,

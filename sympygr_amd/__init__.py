@@ -87,6 +87,8 @@ _SIGS = {
     "sgr_bssnrhs_host": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(SgrParams),
                                         ctypes.c_int]),
+    "sgr_rhs_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_void_p)]),
     "sgr_check_finite": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     "sgr_version": (ctypes.c_char_p, []),
@@ -180,6 +182,14 @@ class Plan:
             raise ValueError("field arrays are shorter than the block table")
         s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
         check(lib().sgr_rhs(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s)))
+
+    def rhs_events(self, out, u, events, stream=None):
+        """sgr_rhs plus hipEvent records around each of the two kernels;
+        `events` is a sequence of 4 raw hipEvent_t handles (ints)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
+        ev = (ctypes.c_void_p * 4)(*events)
+        check(lib().sgr_rhs_events(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s), ev))
 
     def check_finite(self, u, stream=None):
         import torch

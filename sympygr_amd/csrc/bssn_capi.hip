@@ -265,7 +265,8 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
     return SGR_OK;
 }
 
-int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], hipStream_t stream)
+static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
+                    hipStream_t stream, hipEvent_t *ev)
 {
     if (!plan || !out || !in) return fail(SGR_E_INVALID, "null argument");
     LaunchArgs a{};
@@ -283,6 +284,7 @@ int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const
     a.params = device_params(plan->params);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
@@ -290,6 +292,18 @@ int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return SGR_OK;
+}
+
+int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], hipStream_t stream)
+{
+    return rhs_impl(plan, out, in, stream, nullptr);
+}
+
+int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
+                   hipStream_t stream, hipEvent_t events[4])
+{
+    if (!events) return fail(SGR_E_INVALID, "events is null");
+    return rhs_impl(plan, out, in, stream, events);
 }
 
 int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipStream_t stream)

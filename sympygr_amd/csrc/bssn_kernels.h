@@ -17,6 +17,7 @@ struct LaunchArgs {
     Params params;
     bool upwind;
     bool mixed;
+    hipEvent_t ev[4];            // optional: start/stop of k_derivs, start/stop of k_pointwise
 };
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);

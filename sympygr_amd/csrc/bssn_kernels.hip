@@ -20,6 +20,8 @@
 //
 // Both kernels read the block table and a tile->block map, so one launch
 // covers any mix of block sizes.
+#include <hip/hip_ext.h>
+
 #include "bssn_kernels.h"
 #include "stencils.h"
 
@@ -243,12 +245,14 @@ template <typename R, typename W, bool UPWIND>
 static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
 {
     W *ws = (W *)a.ws;
-    hipLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws,
-                       a.wsK, a.np);
+    // hipExtLaunchKernelGGL records the optional events around exactly this
+    // kernel on this stream (null events: plain launch).
+    hipExtLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
+                          a.blocks, a.tile_block, ws, a.wsK, a.np);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_pointwise<R, W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, a.out, a.blocks,
-                       a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
+    hipExtLaunchKernelGGL((k_pointwise<R, W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.ev[2], a.ev[3], 0,
+                          a.in, a.out, a.blocks, a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
     return hipGetLastError();
 }
 

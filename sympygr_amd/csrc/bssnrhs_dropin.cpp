@@ -1,0 +1,54 @@
+// C++ drop-in bssnrhs(...) over the C ABI (see include/bssn_dropin.h).
+#include <atomic>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/bssn_dropin.h"
+#include "../../include/sympygr_amd.h"
+
+// Parameters the reference reads from bssn:: globals at call time
+// (bssn/include/parameters.h, defaults bssn/src/parameters.cpp:99-109).  Weak:
+// an application that defines them (Dendro's parameters.cpp) wins.
+namespace bssn {
+__attribute__((weak)) unsigned int BSSN_LAMBDA[4] = {1, 1, 1, 1};
+__attribute__((weak)) double BSSN_LAMBDA_F[2] = {1.0, 0.0};
+__attribute__((weak)) double ETA_CONST = 2.0;
+__attribute__((weak)) double ETA_R0 = 50.0;
+__attribute__((weak)) double ETA_DAMPING_EXP = 1.0;
+__attribute__((weak)) double KO_DISS_SIGMA = 0.01;
+}  // namespace bssn
+
+namespace {
+std::atomic<int32_t> g_adv{SGR_ADV_CENTRED};
+std::atomic<int32_t> g_prec{SGR_PREC_FP64};
+std::atomic<int> g_device{0};
+}  // namespace
+
+extern "C" void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device)
+{
+    g_adv = adv_mode;
+    g_prec = precision;
+    g_device = device;
+}
+
+void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
+             const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
+{
+    sgr_params p;
+    for (int i = 0; i < 4; i++) p.lambda[i] = bssn::BSSN_LAMBDA[i];
+    p.lambda_f[0] = bssn::BSSN_LAMBDA_F[0];
+    p.lambda_f[1] = bssn::BSSN_LAMBDA_F[1];
+    p.eta_const = bssn::ETA_CONST;
+    p.eta_r0 = bssn::ETA_R0;
+    p.eta_damping_exp = bssn::ETA_DAMPING_EXP;
+    p.ko_sigma = bssn::KO_DISS_SIGMA;
+    p.adv_mode = g_adv;
+    p.precision = g_prec;
+    const uint32_t s[3] = {sz[0], sz[1], sz[2]};
+    const int rc = sgr_bssnrhs_host(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, s, bflag, &p, g_device);
+    if (rc != SGR_OK) {
+        char buf[512];
+        sgr_last_error(buf, sizeof buf);
+        throw std::runtime_error(std::string("bssnrhs: ") + buf);
+    }
+}

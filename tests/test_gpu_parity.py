@@ -123,3 +123,27 @@ def test_host_dropin_matches_device_path():
     got = out[:, 1000:]
     assert np.all(out[:, :1000] == 3.0)
     np.testing.assert_array_equal(interior(got, sz), interior(dev, sz))
+
+
+def test_cpp_dropin_bssnrhs_reads_bssn_globals():
+    """The C++ drop-in (exact reference signature, host arrays, parameters from
+    the bssn:: globals) against the reference fixture."""
+    import ctypes
+    import sympygr_amd as sgr
+    L = ctypes.CDLL(sgr.LIB_PATH)
+    fn = getattr(L, "_Z7bssnrhsPPdPPKdRKjS2_S2_PS4_S5_")
+    PD = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.POINTER(PD), ctypes.POINTER(PD), ctypes.POINTER(ctypes.c_uint), PD, PD,
+                   ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
+    fn.restype = None
+    ctypes.c_double.in_dll(L, "_ZN4bssn13KO_DISS_SIGMAE").value = 0.1
+    L.sgr_dropin_configure(1, 0, 0)
+    u, sz, pmin, pmax, c = case_input("c0_upwind_b63")
+    out = np.zeros_like(u)
+    pi = (PD * 24)(*[u[f].ctypes.data_as(PD) for f in range(24)])
+    po_ = (PD * 24)(*[out[f].ctypes.data_as(PD) for f in range(24)])
+    off, bf = ctypes.c_uint(0), ctypes.c_uint(63)
+    fn(po_, pi, ctypes.byref(off), (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+       (ctypes.c_uint * 3)(*sz), ctypes.byref(bf))
+    err = normwise(interior(out, sz), golden("c0_upwind_b63"))
+    assert err.max() <= FP64_TOL, err
