@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -74,11 +75,6 @@ int validate_block(const sgr_block &b, uint32_t idx)
             snprintf(msg, sizeof msg, "block %u: sz[%d]=%u < 13 (stencils need 7 interior points)", idx, d, b.sz[d]);
             return fail(SGR_E_INVALID, msg);
         }
-        if ((b.sz[d] - 6) % T == 1) {
-            snprintf(msg, sizeof msg,
-                     "block %u: interior size %u = 1 mod %d is not supported by the 8^3 tiling", idx, b.sz[d] - 6, T);
-            return fail(SGR_E_UNSUPPORTED, msg);
-        }
         if (!(b.pmax[d] > b.pmin[d]) || !std::isfinite(b.pmin[d]) || !std::isfinite(b.pmax[d])) {
             snprintf(msg, sizeof msg, "block %u: pmax[%d] must be finite and > pmin[%d]", idx, d, d);
             return fail(SGR_E_INVALID, msg);
@@ -119,7 +115,10 @@ __global__ void k_check_finite(FieldPtrs in, const BlockDesc *blocks, uint32_t n
 
 }  // namespace
 
+enum class Variant { Fused, Workspace };
+
 struct sgr_plan {
+    Variant variant = Variant::Workspace;
     int device = 0;
     sgr_params params{};
     std::vector<BlockDesc> hblocks;
@@ -177,7 +176,17 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     if (!p) return fail(SGR_E_NOMEM, "host allocation failed");
     p->device = device;
     p->params = *params;
-    p->nslots = params->adv_mode == SGR_ADV_UPWIND ? NSLOT_UPWIND : NSLOT_CENTRED;
+    // Default: the two-pass kernels (LDS-tiled derivative kernel into an HBM
+    // workspace, then three output-group pointwise kernels).  SGR_KERNEL=fused
+    // selects the single-pass kernel (no workspace; slower on gfx950 today
+    // because its register footprint allows one wave per SIMD).
+    {
+        const char *kv = getenv("SGR_KERNEL");
+        if (kv && !strcmp(kv, "fused")) p->variant = Variant::Fused;
+    }
+    const bool fusedv = p->variant == Variant::Fused;
+    const int tdim[3] = {fusedv ? FUSED_TX : T, fusedv ? FUSED_TY : T, fusedv ? FUSED_TZ : T};
+    p->nslots = fusedv ? 0 : (params->adv_mode == SGR_ADV_UPWIND ? NSLOT_UPWIND : NSLOT_CENTRED);
     std::vector<uint32_t> tile_block;
     uint64_t ws_base = 0;
     for (uint32_t b = 0; b < nblocks; b++) {
@@ -194,7 +203,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         uint64_t nt = 1;
         for (int k = 0; k < 3; k++) {
             d.n[k] = (int32_t)blocks[b].sz[k];
-            d.ntile[k] = (int32_t)((blocks[b].sz[k] - 6 + T - 1) / T);
+            d.ntile[k] = (int32_t)((blocks[b].sz[k] - 6 + tdim[k] - 1) / tdim[k]);
             d.pmin[k] = blocks[b].pmin[k];
             d.ax[k] = axis_const(blocks[b].pmin[k], blocks[b].pmax[k], blocks[b].sz[k]);
             nt *= (uint64_t)d.ntile[k];
@@ -204,7 +213,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
             return fail(SGR_E_INVALID, "too many tiles");
         }
         for (uint64_t t = 0; t < nt; t++) tile_block.push_back(b);
-        ws_base += nt * TILE_PTS;
+        if (!fusedv) ws_base += nt * TILE_PTS;
         p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
         p->hblocks.push_back(d);
     }
@@ -217,7 +226,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     p->ntiles = (uint32_t)tile_block.size();
     p->np = ws_base;
     const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
-    p->ws_bytes = p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
+    p->ws_bytes = fusedv ? 0 : p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
 
     int cur = 0;
     hipGetDevice(&cur);
@@ -238,9 +247,12 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         if ((e = hipMemcpy(p->d_tile_block, tile_block.data(), tile_block.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(tiles)");
-        if ((e = hipMalloc(&p->d_ws, p->np * (uint64_t)p->nslots * wsz)) != hipSuccess) return bail(e, "hipMalloc(workspace)");
-        if ((e = hipMalloc(&p->d_wsK, p->np * (uint64_t)NF * sizeof(double))) != hipSuccess)
-            return bail(e, "hipMalloc(ko workspace)");
+        if (!fusedv) {
+            if ((e = hipMalloc(&p->d_ws, p->np * (uint64_t)p->nslots * wsz)) != hipSuccess)
+                return bail(e, "hipMalloc(workspace)");
+            if ((e = hipMalloc(&p->d_wsK, p->np * (uint64_t)NF * sizeof(double))) != hipSuccess)
+                return bail(e, "hipMalloc(ko workspace)");
+        }
     }
     if ((e = hipMalloc(&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
     hipSetDevice(cur);
@@ -288,7 +300,15 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = launch_rhs(a, stream);
+    hipError_t e;
+    if (plan->variant == Variant::Fused) {
+        e = launch_fused(a, stream);
+        // one kernel: the second event pair brackets nothing
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
+    } else {
+        e = launch_rhs(a, stream);
+    }
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return SGR_OK;

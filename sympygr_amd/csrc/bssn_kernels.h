@@ -20,6 +20,10 @@ struct LaunchArgs {
     hipEvent_t ev[4];            // optional: start/stop of k_derivs, start/stop of k_pointwise
 };
 
-hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);
+hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);    // derivative workspace + pointwise
+hipError_t launch_fused(const LaunchArgs &a, hipStream_t s);  // fused single pass
+
+// tile shapes (interior points) of the two kernel variants
+constexpr int FUSED_TX = 8, FUSED_TY = 8, FUSED_TZ = 4;
 
 }  // namespace sgr

@@ -13,10 +13,14 @@
 //                derivative workspace (double, or float in mixed mode) and
 //                the KO sums (always double).
 //
-//   k_pointwise  one thread per interior point: reads its fields and
-//                derivatives, runs the generated algebra in R (double or
-//                float), applies the Sommerfeld condition on flagged faces,
-//                adds sigma * KO in double and writes the RHS.
+//   k_pointwise_g{1,2,3}
+//                one thread per interior point, one kernel per output group
+//                (alpha/beta/chi/gt; K/Gt/B; At): reads its fields and
+//                staged derivatives, runs that group's part of the generated
+//                algebra in R (double or float), applies the Sommerfeld
+//                condition on flagged faces, adds sigma * KO in double and
+//                writes the RHS.  Splitting by group keeps each kernel's
+//                register file small enough for several waves per SIMD.
 //
 // Both kernels read the block table and a tile->block map, so one launch
 // covers any mix of block sizes.
@@ -43,18 +47,31 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
     const int tx = c.lt % B.ntile[0];
     const int ty = (c.lt / B.ntile[0]) % B.ntile[1];
     const int tz = c.lt / (B.ntile[0] * B.ntile[1]);
-    c.i0 = GW + T * tx;
-    c.j0 = GW + T * ty;
-    c.k0 = GW + T * tz;
+    // last tile shifted back to end at the interior's end (overlap is benign:
+    // every tile owns its own workspace points and writes identical results)
+    auto origin = [](int tt, int n) {
+        const int o = GW + T * tt, last = n - GW - T;
+        return o < last ? o : (last > GW ? last : GW);
+    };
+    c.i0 = origin(tx, B.n[0]);
+    c.j0 = origin(ty, B.n[1]);
+    c.k0 = origin(tz, B.n[2]);
     return c;
 }
+
+// Fields whose first derivatives the pointwise algebra never reads (B, At):
+// with upwinded advection their first derivatives are only needed by the
+// Sommerfeld condition, i.e. on tiles that touch a flagged face.
+__host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && f <= 11) || f >= 18; }
 
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                 double *__restrict__ wsK, uint64_t NP)
 {
-    __shared__ double U[TH * TH * TH];   // x + 14 (y + 14 z), halo coordinates
+    constexpr int NHALO = TH * TH * TH;            // 2744
+    constexpr int NPF = (NHALO + 255) / 256;       // 11 prefetch registers per thread
+    __shared__ double U[NHALO];          // x + 14 (y + 14 z), halo coordinates
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
 
@@ -65,6 +82,9 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
     const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
     const AxisConst a0 = B.ax[0], a1 = B.ax[1], a2 = B.ax[2];
+    // does this tile contain a point of a flagged face? (uniform)
+    const bool face_tile = (lo0 && tc.i0 == GW) || (hi0 && tc.i0 + T >= n0 - GW) || (lo1 && tc.j0 == GW) ||
+                           (hi1 && tc.j0 + T >= n1 - GW) || (lo2 && tc.k0 == GW) || (hi2 && tc.k0 + T >= n2 - GW);
 
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;  // lz in [0,4); points lz and lz+4
@@ -72,7 +92,7 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     bool valid[2];
     int pk[2];
     uint64_t wp[2];
-    double beta[2][3];
+    bool bpos[2][3];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         pk[p] = tc.k0 + lz + 4 * p;
@@ -80,30 +100,48 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
         wp[p] = B.ws_base + (uint64_t)tc.lt * TILE_PTS + (lx + T * (ly + T * (lz + 4 * p)));
         const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) beta[p][d] = (UPWIND && valid[p]) ? in.p[6 + d][gp] : 0.0;
+        for (int d = 0; d < 3; d++) bpos[p][d] = (UPWIND && valid[p]) ? in.p[6 + d][gp] > 0.0 : false;
     }
+    // halo element -> offset in the block (or -1 outside the padded block)
+    int32_t src[NPF];
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+        const int e = t + 256 * q;
+        const int ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
+        const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
+        src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
+    }
+    {
+        const double *__restrict__ s0 = in.p[0] + B.offset;
+#pragma unroll
+        for (int q = 0; q < NPF; q++)
+            if (t + 256 * q < NHALO) U[t + 256 * q] = src[q] >= 0 ? s0[src[q]] : 0.0;
+    }
+    __syncthreads();
 
     for (int f = 0; f < NF; f++) {
-        __syncthreads();
-        const double *__restrict__ src = in.p[f] + B.offset;
-        for (int e = t; e < TH * TH * TH; e += 256) {
-            const int ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
-            const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
-            U[e] = (gi < n0 && gj < n1 && gk < n2) ? src[gi + (uint64_t)n0 * (gj + (uint64_t)n1 * gk)] : 0.0;
+        // fetch the next field's halo tile into registers; it lands in LDS
+        // after this field's stencils, so the HBM latency overlaps them
+        double pf[NPF];
+        if (f + 1 < NF) {
+            const double *__restrict__ sn = in.p[f + 1] + B.offset;
+#pragma unroll
+            for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
         }
-        __syncthreads();
-
+        const bool need_d1 = !UPWIND || !d1_only_for_faces(f) || face_tile;
 #pragma unroll
         for (int p = 0; p < 2; p++) {
             if (!valid[p]) continue;
             const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-            ws[slot_d1(f, 0) * NP + wp[p]] = (W)d1(c, 1, pi, n0, lo0, hi0, a0);
-            ws[slot_d1(f, 1) * NP + wp[p]] = (W)d1(c, TH, pj, n1, lo1, hi1, a1);
-            ws[slot_d1(f, 2) * NP + wp[p]] = (W)d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+            if (need_d1) {
+                ws[slot_d1(f, 0) * NP + wp[p]] = (W)d1(c, 1, pi, n0, lo0, hi0, a0);
+                ws[slot_d1(f, 1) * NP + wp[p]] = (W)d1(c, TH, pj, n1, lo1, hi1, a1);
+                ws[slot_d1(f, 2) * NP + wp[p]] = (W)d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+            }
             if (UPWIND) {
-                ws[slot_ad(f, 0) * NP + wp[p]] = (W)adv(c, 1, pi, n0, lo0, hi0, beta[p][0], a0);
-                ws[slot_ad(f, 1) * NP + wp[p]] = (W)adv(c, TH, pj, n1, lo1, hi1, beta[p][1], a1);
-                ws[slot_ad(f, 2) * NP + wp[p]] = (W)adv(c, TH * TH, pk[p], n2, lo2, hi2, beta[p][2], a2);
+                ws[slot_ad(f, 0) * NP + wp[p]] = (W)adv(c, 1, pi, n0, lo0, hi0, bpos[p][0] ? 1.0 : 0.0, a0);
+                ws[slot_ad(f, 1) * NP + wp[p]] = (W)adv(c, TH, pj, n1, lo1, hi1, bpos[p][1] ? 1.0 : 0.0, a1);
+                ws[slot_ad(f, 2) * NP + wp[p]] = (W)adv(c, TH * TH, pk[p], n2, lo2, hi2, bpos[p][2] ? 1.0 : 0.0, a2);
             }
             {
                 SGR_EXACT
@@ -114,42 +152,50 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
             }
         }
 
-        if (d2q(f) < 0) continue;  // uniform over the workgroup
-
+        if (d2q(f) >= 0) {  // uniform over the workgroup
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            if (!valid[p]) continue;
-            const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-            ws[slot_d2(f, 0) * NP + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
-            ws[slot_d2(f, 3) * NP + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
-            ws[slot_d2(f, 5) * NP + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+            for (int p = 0; p < 2; p++) {
+                if (!valid[p]) continue;
+                const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                ws[slot_d2(f, 0) * NP + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
+                ws[slot_d2(f, 3) * NP + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
+                ws[slot_d2(f, 5) * NP + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+            }
+            // inner derivatives of the mixed terms, valid wherever the block has them
+            for (int e = t; e < T * TH * TH; e += 256) {
+                const int x = e % T, y = (e / T) % TH, z = e / (T * TH);
+                const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
+                double v = 0.0;
+                if (gi < n0 - GW && gj < n1 && gk < n2)
+                    v = d1(&U[(x + GW) + TH * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
+                DX[e] = v;
+            }
+            for (int e = t; e < T * T * TH; e += 256) {
+                const int x = e % T, y = (e / T) % T, z = e / (T * T);
+                const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
+                double v = 0.0;
+                if (gi < n0 - GW && gj < n1 - GW && gk < n2)
+                    v = d1(&U[(x + GW) + TH * ((y + GW) + TH * z)], TH, gj, n1, lo1, hi1, a1);
+                DY[e] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                if (!valid[p]) continue;
+                const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
+                ws[slot_d2(f, 1) * NP + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
+                ws[slot_d2(f, 2) * NP + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
+                ws[slot_d2(f, 4) * NP + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
+            }
         }
-        // inner derivatives of the mixed terms, valid wherever the block has them
-        for (int e = t; e < T * TH * TH; e += 256) {
-            const int x = e % T, y = (e / T) % TH, z = e / (T * TH);
-            const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
-            double v = 0.0;
-            if (gi < n0 - GW && gj < n1 && gk < n2) v = d1(&U[(x + GW) + TH * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
-            DX[e] = v;
-        }
-        for (int e = t; e < T * T * TH; e += 256) {
-            const int x = e % T, y = (e / T) % T, z = e / (T * T);
-            const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
-            double v = 0.0;
-            if (gi < n0 - GW && gj < n1 - GW && gk < n2)
-                v = d1(&U[(x + GW) + TH * ((y + GW) + TH * z)], TH, gj, n1, lo1, hi1, a1);
-            DY[e] = v;
+        __syncthreads();  // every read of U (and DX, DY) for field f is done
+        if (f + 1 < NF) {
+#pragma unroll
+            for (int q = 0; q < NPF; q++)
+                if (t + 256 * q < NHALO) U[t + 256 * q] = pf[q];
         }
         __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            if (!valid[p]) continue;
-            const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
-            const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
-            ws[slot_d2(f, 1) * NP + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
-            ws[slot_d2(f, 2) * NP + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
-            ws[slot_d2(f, 4) * NP + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
-        }
     }
 }
 
@@ -180,65 +226,21 @@ __device__ __forceinline__ double ko_add(double v, double sigma, double k)
     return v + sigma * k;
 }
 
-template <typename R, typename W, bool UPWIND>
-__global__ __launch_bounds__(256) void k_pointwise(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
-                                                   const uint32_t *__restrict__ tile_block,
-                                                   const W *__restrict__ ws, const double *__restrict__ wsK,
-                                                   uint64_t NP, Params P)
-{
-    const uint32_t tile = blockIdx.x >> 1;
-    const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
-    const BlockDesc &B = blocks[tile_block[tile]];
-    const TileCoord tc = tile_coord(B, tile);
-    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-    const int i = tc.i0 + lx, j = tc.j0 + ly, k = tc.k0 + lz;
-    const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-    if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
-    const uint64_t wp = B.ws_base + (uint64_t)tc.lt * TILE_PTS + l;
-    const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
-
-    // coordinates and eta exactly as bssn/src/rhs.cpp:206-219
-    const double x = coord(B.pmin[0], i, B.ax[0].h);
-    const double y = coord(B.pmin[1], j, B.ax[1].h);
-    const double z = coord(B.pmin[2], k, B.ax[2].h);
-    double eta_d = P.eta_const;
-    {
-        const double r_coord = radius(x, y, z);
-        if (r_coord >= P.eta_r0) eta_d *= pow(P.eta_r0 / r_coord, P.eta_damping_exp);
-    }
-    const uint32_t bf = B.bflag;
-    const bool face = bf && (((bf & 1u) && i == GW) || ((bf & 2u) && i == n0 - GW - 1) ||
-                             ((bf & 4u) && j == GW) || ((bf & 8u) && j == n1 - GW - 1) ||
-                             ((bf & 16u) && k == GW) || ((bf & 32u) && k == n2 - GW - 1));
-    const double inv_r = face ? 1.0 / radius(x, y, z) : 0.0;
-
-    const R lambda0 = (R)P.lambda[0], lambda1 = (R)P.lambda[1], lambda2 = (R)P.lambda[2], lambda3 = (R)P.lambda[3];
-    const R lambda_f0 = (R)P.lambda_f[0], lambda_f1 = (R)P.lambda_f[1];
-    const R eta = (R)eta_d;
-    const double sigma = P.ko_sigma;
-
-#define BSSN_U(F_) ((R)in.p[F_][pp])
-#define BSSN_D1(F_, D_) ((R)ws[slot_d1(F_, D_) * NP + wp])
-#define BSSN_D2(F_, S_) ((R)ws[slot_d2(F_, S_) * NP + wp])
-#define BSSN_AD(F_, D_) ((R)ws[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * NP + wp])
-#define BSSN_OUT(F_, V_)                                                                                      \
-    do {                                                                                                    \
-        double val_ = (double)(V_);                                                                          \
-        if (face)                                                                                           \
-            val_ = sommerfeld(x, y, z, inv_r, (double)ws[slot_d1(F_, 0) * NP + wp],                          \
-                              (double)ws[slot_d1(F_, 1) * NP + wp], (double)ws[slot_d1(F_, 2) * NP + wp],     \
-                              in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                      \
-        out.p[F_][pp] = ko_add(val_, sigma, wsK[(uint64_t)(F_) * NP + wp]);                                   \
-    } while (0)
-#define BSSN_BARRIER() SGR_ALGEBRA_BARRIER
-#include "gen/bssn_algebra.inc"
-#undef BSSN_BARRIER
-#undef BSSN_U
-#undef BSSN_D1
-#undef BSSN_D2
-#undef BSSN_AD
-#undef BSSN_OUT
-}
+#define SGR_PW_NAME k_pointwise_g1
+#define SGR_PW_ALG "gen/bssn_algebra_g1.inc"
+#include "pointwise_body.h"
+#undef SGR_PW_NAME
+#undef SGR_PW_ALG
+#define SGR_PW_NAME k_pointwise_g2
+#define SGR_PW_ALG "gen/bssn_algebra_g2.inc"
+#include "pointwise_body.h"
+#undef SGR_PW_NAME
+#undef SGR_PW_ALG
+#define SGR_PW_NAME k_pointwise_g3
+#define SGR_PW_ALG "gen/bssn_algebra_g3.inc"
+#include "pointwise_body.h"
+#undef SGR_PW_NAME
+#undef SGR_PW_ALG
 
 // ------------------------------------------------------------------ launch
 template <typename R, typename W, bool UPWIND>
@@ -251,8 +253,13 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
                           a.blocks, a.tile_block, ws, a.wsK, a.np);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipExtLaunchKernelGGL((k_pointwise<R, W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.ev[2], a.ev[3], 0,
-                          a.in, a.out, a.blocks, a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
+    const dim3 g(2 * a.ntiles), b(256);
+    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
+    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND>), g, b, 0, s, nullptr, nullptr, 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
+    hipExtLaunchKernelGGL((k_pointwise_g3<R, W, UPWIND>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
     return hipGetLastError();
 }
 

@@ -1,0 +1,66 @@
+// Body of one output-group pointwise kernel; included once per group with
+// SGR_PW_NAME (kernel name) and SGR_PW_ALG (generated algebra file) defined.
+// One thread per interior point: reads its fields and staged derivatives,
+// runs the group's part of the generated algebra in R (double or float),
+// applies the Sommerfeld condition on flagged faces and adds sigma * KO in
+// double.
+template <typename R, typename W, bool UPWIND>
+__global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
+                                                   const uint32_t *__restrict__ tile_block,
+                                                   const W *__restrict__ ws, const double *__restrict__ wsK,
+                                                   uint64_t NP, Params P)
+{
+    const uint32_t tile = blockIdx.x >> 1;
+    const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
+    const BlockDesc &B = blocks[tile_block[tile]];
+    const TileCoord tc = tile_coord(B, tile);
+    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int i = tc.i0 + lx, j = tc.j0 + ly, k = tc.k0 + lz;
+    const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+    if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
+    const uint64_t wp = B.ws_base + (uint64_t)tc.lt * TILE_PTS + l;
+    const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
+
+    // coordinates and eta exactly as bssn/src/rhs.cpp:206-219
+    const double x = coord(B.pmin[0], i, B.ax[0].h);
+    const double y = coord(B.pmin[1], j, B.ax[1].h);
+    const double z = coord(B.pmin[2], k, B.ax[2].h);
+    double eta_d = P.eta_const;
+    {
+        const double r_coord = radius(x, y, z);
+        if (r_coord >= P.eta_r0) eta_d *= pow(P.eta_r0 / r_coord, P.eta_damping_exp);
+    }
+    const uint32_t bf = B.bflag;
+    const bool face = bf && (((bf & 1u) && i == GW) || ((bf & 2u) && i == n0 - GW - 1) ||
+                             ((bf & 4u) && j == GW) || ((bf & 8u) && j == n1 - GW - 1) ||
+                             ((bf & 16u) && k == GW) || ((bf & 32u) && k == n2 - GW - 1));
+    const double inv_r = face ? 1.0 / radius(x, y, z) : 0.0;
+
+    const R lambda0 = (R)P.lambda[0], lambda1 = (R)P.lambda[1], lambda2 = (R)P.lambda[2], lambda3 = (R)P.lambda[3];
+    const R lambda_f0 = (R)P.lambda_f[0], lambda_f1 = (R)P.lambda_f[1];
+    const R eta = (R)eta_d;
+    const double sigma = P.ko_sigma;
+
+#define BSSN_U(F_) ((R)in.p[F_][pp])
+#define BSSN_D1(F_, D_) ((R)ws[slot_d1(F_, D_) * NP + wp])
+#define BSSN_D2(F_, S_) ((R)ws[slot_d2(F_, S_) * NP + wp])
+#define BSSN_AD(F_, D_) ((R)ws[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * NP + wp])
+#define BSSN_OUT(F_, V_)                                                                                      \
+    do {                                                                                                    \
+        double val_ = (double)(V_);                                                                          \
+        if (face)                                                                                           \
+            val_ = sommerfeld(x, y, z, inv_r, (double)ws[slot_d1(F_, 0) * NP + wp],                          \
+                              (double)ws[slot_d1(F_, 1) * NP + wp], (double)ws[slot_d1(F_, 2) * NP + wp],     \
+                              in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                      \
+        out.p[F_][pp] = ko_add(val_, sigma, wsK[(uint64_t)(F_) * NP + wp]);                                   \
+    } while (0)
+#define BSSN_BARRIER() SGR_ALGEBRA_BARRIER
+#include SGR_PW_ALG
+#undef BSSN_BARRIER
+#undef BSSN_U
+#undef BSSN_D1
+#undef BSSN_D2
+#undef BSSN_AD
+#undef BSSN_OUT
+}
+
