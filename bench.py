@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference point for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the RHS path is single-GPU here); got --gpus {args.gpus}",
@@ -221,6 +223,13 @@ def main():
         "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                         "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
     }
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle on a bounded sample)")

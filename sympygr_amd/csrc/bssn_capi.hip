@@ -224,7 +224,15 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         return derr != hipSuccess ? hip_fail(derr, "hipGetDeviceCount") : fail(SGR_E_INVALID, "device index out of range");
     }
     p->ntiles = (uint32_t)tile_block.size();
-    p->np = ws_base;
+    // Plane stride: the point count padded by a few KiB so consecutive slot
+    // planes do not start at the same large power-of-two alignment (which
+    // maps every plane of a wave's reads onto the same HBM channels).
+    {
+        uint64_t pad = 2048 + 64;
+        const char *pv = getenv("SGR_WS_PAD");
+        if (pv) pad = strtoull(pv, nullptr, 10);
+        p->np = ws_base + (ws_base ? pad : 0);
+    }
     const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
     p->ws_bytes = fusedv ? 0 : p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
 

@@ -11,9 +11,9 @@ struct LaunchArgs {
     const BlockDesc *blocks;     // device
     const uint32_t *tile_block;  // device, block id of every tile
     uint32_t ntiles;
-    void *ws;                    // derivative workspace, nslot x np (double or float)
-    double *wsK;                 // KO sums, 24 x np
-    uint64_t np;                 // workspace points (ntiles * 512)
+    void *ws;                    // derivative workspace [slot][np padded to the stride] (double or float)
+    double *wsK;                 // KO sums [24][stride]
+    uint64_t np;                 // plane stride in points (>= ntiles * 512)
     Params params;
     bool upwind;
     bool mixed;

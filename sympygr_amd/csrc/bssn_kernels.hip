@@ -67,7 +67,7 @@ __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && 
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
-                                                double *__restrict__ wsK, uint64_t NP)
+                                                double *__restrict__ wsK, uint64_t SS)
 {
     constexpr int NHALO = TH * TH * TH;            // 2744
     constexpr int NPF = (NHALO + 255) / 256;       // 11 prefetch registers per thread
@@ -93,11 +93,15 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     int pk[2];
     uint64_t wp[2];
     bool bpos[2][3];
+    // slot-major workspace with a padded plane stride SS (points); this tile's
+    // 512 points are contiguous within every plane
+    W *__restrict__ wsb = ws + (uint64_t)tile * TILE_PTS;
+    double *__restrict__ kb = wsK + (uint64_t)tile * TILE_PTS;
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         pk[p] = tc.k0 + lz + 4 * p;
         valid[p] = pi < n0 - GW && pj < n1 - GW && pk[p] < n2 - GW;
-        wp[p] = B.ws_base + (uint64_t)tc.lt * TILE_PTS + (lx + T * (ly + T * (lz + 4 * p)));
+        wp[p] = (uint64_t)(lx + T * (ly + T * (lz + 4 * p)));
         const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
 #pragma unroll
         for (int d = 0; d < 3; d++) bpos[p][d] = (UPWIND && valid[p]) ? in.p[6 + d][gp] > 0.0 : false;
@@ -134,21 +138,21 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
             if (!valid[p]) continue;
             const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
             if (need_d1) {
-                ws[slot_d1(f, 0) * NP + wp[p]] = (W)d1(c, 1, pi, n0, lo0, hi0, a0);
-                ws[slot_d1(f, 1) * NP + wp[p]] = (W)d1(c, TH, pj, n1, lo1, hi1, a1);
-                ws[slot_d1(f, 2) * NP + wp[p]] = (W)d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                wsb[slot_d1(f, 0) * SS + wp[p]] = (W)d1(c, 1, pi, n0, lo0, hi0, a0);
+                wsb[slot_d1(f, 1) * SS + wp[p]] = (W)d1(c, TH, pj, n1, lo1, hi1, a1);
+                wsb[slot_d1(f, 2) * SS + wp[p]] = (W)d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
             }
             if (UPWIND) {
-                ws[slot_ad(f, 0) * NP + wp[p]] = (W)adv(c, 1, pi, n0, lo0, hi0, bpos[p][0] ? 1.0 : 0.0, a0);
-                ws[slot_ad(f, 1) * NP + wp[p]] = (W)adv(c, TH, pj, n1, lo1, hi1, bpos[p][1] ? 1.0 : 0.0, a1);
-                ws[slot_ad(f, 2) * NP + wp[p]] = (W)adv(c, TH * TH, pk[p], n2, lo2, hi2, bpos[p][2] ? 1.0 : 0.0, a2);
+                wsb[slot_ad(f, 0) * SS + wp[p]] = (W)adv(c, 1, pi, n0, lo0, hi0, bpos[p][0] ? 1.0 : 0.0, a0);
+                wsb[slot_ad(f, 1) * SS + wp[p]] = (W)adv(c, TH, pj, n1, lo1, hi1, bpos[p][1] ? 1.0 : 0.0, a1);
+                wsb[slot_ad(f, 2) * SS + wp[p]] = (W)adv(c, TH * TH, pk[p], n2, lo2, hi2, bpos[p][2] ? 1.0 : 0.0, a2);
             }
             {
                 SGR_EXACT
                 const double kx = ko(c, 1, pi, n0, lo0, hi0, a0);
                 const double ky = ko(c, TH, pj, n1, lo1, hi1, a1);
                 const double kz = ko(c, TH * TH, pk[p], n2, lo2, hi2, a2);
-                wsK[(uint64_t)f * NP + wp[p]] = kx + ky + kz;
+                kb[f * SS + wp[p]] = kx + ky + kz;
             }
         }
 
@@ -157,9 +161,9 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
             for (int p = 0; p < 2; p++) {
                 if (!valid[p]) continue;
                 const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                ws[slot_d2(f, 0) * NP + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
-                ws[slot_d2(f, 3) * NP + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
-                ws[slot_d2(f, 5) * NP + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                wsb[slot_d2(f, 0) * SS + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
+                wsb[slot_d2(f, 3) * SS + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
+                wsb[slot_d2(f, 5) * SS + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
             for (int e = t; e < T * TH * TH; e += 256) {
@@ -184,9 +188,9 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
                 if (!valid[p]) continue;
                 const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
                 const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
-                ws[slot_d2(f, 1) * NP + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
-                ws[slot_d2(f, 2) * NP + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
-                ws[slot_d2(f, 4) * NP + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
+                wsb[slot_d2(f, 1) * SS + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
+                wsb[slot_d2(f, 2) * SS + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
+                wsb[slot_d2(f, 4) * SS + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
             }
         }
         __syncthreads();  // every read of U (and DX, DY) for field f is done

@@ -8,7 +8,7 @@ template <typename R, typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
                                                    const uint32_t *__restrict__ tile_block,
                                                    const W *__restrict__ ws, const double *__restrict__ wsK,
-                                                   uint64_t NP, Params P)
+                                                   uint64_t SS, Params P)
 {
     const uint32_t tile = blockIdx.x >> 1;
     const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
@@ -18,7 +18,9 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
     const int i = tc.i0 + lx, j = tc.j0 + ly, k = tc.k0 + lz;
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
-    const uint64_t wp = B.ws_base + (uint64_t)tc.lt * TILE_PTS + l;
+    // slot-major workspace, plane stride SS points
+    const W *__restrict__ wsb = ws + (uint64_t)tile * TILE_PTS + l;
+    const double *__restrict__ kb = wsK + (uint64_t)tile * TILE_PTS + l;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
 
     // coordinates and eta exactly as bssn/src/rhs.cpp:206-219
@@ -42,17 +44,17 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
     const double sigma = P.ko_sigma;
 
 #define BSSN_U(F_) ((R)in.p[F_][pp])
-#define BSSN_D1(F_, D_) ((R)ws[slot_d1(F_, D_) * NP + wp])
-#define BSSN_D2(F_, S_) ((R)ws[slot_d2(F_, S_) * NP + wp])
-#define BSSN_AD(F_, D_) ((R)ws[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * NP + wp])
+#define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
+#define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
+#define BSSN_AD(F_, D_) ((R)wsb[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * SS])
 #define BSSN_OUT(F_, V_)                                                                                      \
     do {                                                                                                    \
         double val_ = (double)(V_);                                                                          \
         if (face)                                                                                           \
-            val_ = sommerfeld(x, y, z, inv_r, (double)ws[slot_d1(F_, 0) * NP + wp],                          \
-                              (double)ws[slot_d1(F_, 1) * NP + wp], (double)ws[slot_d1(F_, 2) * NP + wp],     \
+            val_ = sommerfeld(x, y, z, inv_r, (double)wsb[slot_d1(F_, 0) * SS],                          \
+                              (double)wsb[slot_d1(F_, 1) * SS], (double)wsb[slot_d1(F_, 2) * SS],     \
                               in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                      \
-        out.p[F_][pp] = ko_add(val_, sigma, wsK[(uint64_t)(F_) * NP + wp]);                                   \
+        out.p[F_][pp] = ko_add(val_, sigma, kb[(F_) * SS]);                                   \
     } while (0)
 #define BSSN_BARRIER() SGR_ALGEBRA_BARRIER
 #include SGR_PW_ALG
