@@ -101,6 +101,47 @@ int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const
 int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
                    hipStream_t stream, hipEvent_t events[4]);
 
+/* Hamiltonian constraint (out[0]) and momentum constraints (out[1..3]) at
+ * every interior point of the plan's blocks, in FP64 (GR/constraints.py:
+ * 161-173, same derivative operators as the RHS).  Uses the plan's
+ * workspace: do not overlap with sgr_rhs on the same plan.  The reference's
+ * generated counterpart (bssn/CodeGen/physcon.cpp) does not compile. */
+int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream);
+
+/* Same-level ghost refresh (the unzip step between RK stages): every ghost
+ * point of every plan block whose neighbour region is another plan block of
+ * the same size and spacing is copied from that block's interior; ghost
+ * regions beyond flagged (domain-boundary) faces are left as they are (no
+ * stencil reads them).  SGR_E_UNSUPPORTED if the plan has a level boundary
+ * (2:1 prolongation is not provided).  No reference counterpart (Dendro's
+ * unzip lives outside the reference repository). */
+int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t stream);
+
+/* One classic-RK4 stage with the update fused into the RHS output store
+ * (interior points only; a ghost refresh must precede the next stage):
+ *   stage 0: k = RHS(u_stage); acc = u_n + dt/6 k; u_next = u_n + dt/2 k
+ *   stage 1: k = RHS(u_stage); acc += dt/3 k;      u_next = u_n + dt/2 k
+ *   stage 2: k = RHS(u_stage); acc += dt/3 k;      u_next = u_n + dt k
+ *   stage 3: k = RHS(u_stage); u_next = acc + dt/6 k   (u_next may be u_n)
+ * u_next must not alias u_stage.  The reference has only the RK enum
+ * (bssn/include/grDef.h:25); the stage is the textbook tableau. */
+int sgr_rk4_stage(sgr_plan *plan, int stage, double dt, const double *const u_n[SGR_NUM_VARS],
+                  const double *const u_stage[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS],
+                  double *const u_next[SGR_NUM_VARS], hipStream_t stream);
+
+/* A full RK4 step of u in place: ghost refresh + 4 fused stages, using the
+ * caller's stage buffers s_a, s_b and accumulator acc (all same layout). */
+int sgr_rk4_step(sgr_plan *plan, double dt, double *const u[SGR_NUM_VARS], double *const s_a[SGR_NUM_VARS],
+                 double *const s_b[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS], hipStream_t stream);
+
+/* The same step captured once into a HIP graph (4 ghost refreshes + 4 stages =
+ * 20 kernel nodes) and replayed with sgr_graph_launch. */
+typedef struct sgr_graph sgr_graph;
+int sgr_rk4_graph_create(sgr_plan *plan, double dt, double *const u[SGR_NUM_VARS], double *const s_a[SGR_NUM_VARS],
+                         double *const s_b[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS], sgr_graph **graph);
+int sgr_graph_launch(sgr_graph *graph, hipStream_t stream);
+int sgr_graph_destroy(sgr_graph *graph);
+
 /* One block, DEVICE pointers, synchronous: bssnrhs (bssn/include/rhs.h:33-36)
  * with the arrays already on the device. */
 int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,

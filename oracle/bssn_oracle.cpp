@@ -115,8 +115,6 @@ void for_box(const Grid &g, const Box &b, int axis, int pos, F f)
             for (int i = c.lo[0]; i < c.hi[0]; i++) f(g.idx(i, j, k), i, j, k);
 }
 
-int coord(int axis, int i, int j, int k) { return axis == 0 ? i : (axis == 1 ? j : k); }
-
 // deriv64_{x,y,z}: x sweeps i in [3,n-3) and all j,k; y sweeps i,j interior and
 // all k; z sweeps the interior (derivs.cpp:2066-2071, 2197-2202, 2324-2329).
 // The wider ranges give the mixed derivatives valid inner values in ghost rows.
@@ -236,6 +234,28 @@ struct Workspace {
     }
 };
 
+// derivative sweeps (bssnrhs_derivs.h): d_x, d_y, d_z of all fields, pure
+// second derivatives and the mixed ones as compositions grad2_0_1 = D_y(grad_0),
+// grad2_0_2 = D_z(grad_0), grad2_1_2 = D_z(grad_1); advective derivatives along
+// d use beta^d (bssnrhs_derivs_adv.h).
+void sweep_all(Workspace &W, const double *const *in, const double *h, const Grid &g, unsigned bflag, bool upwind)
+{
+    for (int f = 0; f < 24; f++)
+        for (int a = 0; a < 3; a++) first(W.grad[f][a], in[f], h[a], g, bflag, a);
+    for (int q = 0; q < 11; q++) {
+        const int f = D2_FIELDS[q];
+        second(W.grad2[f][0], in[f], h[0], g, bflag, 0);
+        second(W.grad2[f][3], in[f], h[1], g, bflag, 1);
+        second(W.grad2[f][5], in[f], h[2], g, bflag, 2);
+        first(W.grad2[f][1], W.grad[f][0], h[1], g, bflag, 1);
+        first(W.grad2[f][2], W.grad[f][0], h[2], g, bflag, 2);
+        first(W.grad2[f][4], W.grad[f][1], h[2], g, bflag, 2);
+    }
+    if (upwind)
+        for (int f = 0; f < 24; f++)
+            for (int a = 0; a < 3; a++) advective(W.agrad[f][a], in[f], h[a], g, in[6 + a], bflag, a);
+}
+
 int rhs_block(double **outv, const double **inv, uint64_t offset, const double *pmin, const double *pmax,
               const uint32_t *sz, uint32_t bflag, const oracle_params *P, Workspace &W)
 {
@@ -253,25 +273,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
     const double h[3] = {(pmax[0] - pmin[0]) / (g.nx - 1), (pmax[1] - pmin[1]) / (g.ny - 1), (pmax[2] - pmin[2]) / (g.nz - 1)};
     W.resize(n, P->upwind != 0);
 
-    // derivative sweeps (bssnrhs_derivs.h): d_x, d_y, d_z of all fields,
-    // pure second derivatives and the mixed ones as compositions
-    // grad2_0_1 = D_y(grad_0), grad2_0_2 = D_z(grad_0), grad2_1_2 = D_z(grad_1).
-    for (int f = 0; f < 24; f++)
-        for (int a = 0; a < 3; a++) first(W.grad[f][a], in[f], h[a], g, bflag, a);
-    for (int q = 0; q < 11; q++) {
-        const int f = D2_FIELDS[q];
-        second(W.grad2[f][0], in[f], h[0], g, bflag, 0);
-        second(W.grad2[f][3], in[f], h[1], g, bflag, 1);
-        second(W.grad2[f][5], in[f], h[2], g, bflag, 2);
-        first(W.grad2[f][1], W.grad[f][0], h[1], g, bflag, 1);
-        first(W.grad2[f][2], W.grad[f][0], h[2], g, bflag, 2);
-        first(W.grad2[f][4], W.grad[f][1], h[2], g, bflag, 2);
-    }
-    if (P->upwind) {
-        // advective derivative along d uses beta^d (bssnrhs_derivs_adv.h)
-        for (int f = 0; f < 24; f++)
-            for (int a = 0; a < 3; a++) advective(W.agrad[f][a], in[f], h[a], g, in[6 + a], bflag, a);
-    }
+    sweep_all(W, in, h, g, bflag, P->upwind != 0);
 
     const double lambda0 = P->lambda[0], lambda1 = P->lambda[1], lambda2 = P->lambda[2], lambda3 = P->lambda[3];
     const double lambda_f0 = P->lambda_f[0], lambda_f1 = P->lambda_f[1];
@@ -357,6 +359,49 @@ extern "C" int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t n
             err |= rhs_block(out, in, offsets[b], pmin + 3 * b, pmax + 3 * b, sz + 3 * b, bflag[b], p, W) ? 1 : 0;
     }
     return err ? -1 : 0;
+}
+
+// Hamiltonian and momentum constraints at the interior points of one block
+// (GR/constraints.py:161-173), from the same derivative sweeps.
+extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t offset, const double *pmin,
+                                  const double *pmax, const uint32_t *sz, uint32_t bflag)
+{
+    if (!outv || !inv || !pmin || !pmax || !sz) return -1;
+    if (sz[0] < 13 || sz[1] < 13 || sz[2] < 13 || bflag > 63) return -2;
+    const Grid g{(int)sz[0], (int)sz[1], (int)sz[2]};
+    const std::size_t n = (std::size_t)g.nx * g.ny * g.nz;
+    const double *in[24];
+    double *out[4];
+    for (int f = 0; f < 24; f++) {
+        if (!inv[f]) return -1;
+        in[f] = inv[f] + offset;
+    }
+    for (int c = 0; c < 4; c++) {
+        if (!outv[c]) return -1;
+        out[c] = outv[c] + offset;
+    }
+    const double h[3] = {(pmax[0] - pmin[0]) / (g.nx - 1), (pmax[1] - pmin[1]) / (g.ny - 1), (pmax[2] - pmin[2]) / (g.nz - 1)};
+    Workspace W;
+    W.resize(n, false);
+    sweep_all(W, in, h, g, bflag, false);
+    typedef double R;
+    for (int k = 3; k < g.nz - 3; k++)
+        for (int j = 3; j < g.ny - 3; j++)
+            for (int i = 3; i < g.nx - 3; i++) {
+                const std::ptrdiff_t pp = g.idx(i, j, k);
+#define BSSN_U(f) in[f][pp]
+#define BSSN_D1(f, d) W.grad[f][d][pp]
+#define BSSN_D2(f, s) W.grad2[f][s][pp]
+#define BSSN_OUT(c, v) out[c][pp] = (v)
+#define BSSN_BARRIER()
+#include "gen/bssn_constraints.inc"
+#undef BSSN_U
+#undef BSSN_D1
+#undef BSSN_D2
+#undef BSSN_OUT
+#undef BSSN_BARRIER
+            }
+    return 0;
 }
 
 extern "C" int oracle_deriv(int which, double *out, const double *u, double h, const uint32_t *sz, const double *beta,

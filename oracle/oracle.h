@@ -40,6 +40,11 @@ int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t nblocks,
                           const uint32_t *bflag, const oracle_params *p,
                           int nthreads);
 
+/* Hamiltonian constraint (out[0]) and momentum constraints (out[1..3]) at
+ * the interior points of one block, GR/constraints.py:161-173. */
+int oracle_constraints(double **out, const double **in, uint64_t offset, const double *pmin, const double *pmax,
+                       const uint32_t *sz, uint32_t bflag);
+
 /* One stencil sweep: which = 0..2 first x/y/z, 3..5 second xx/yy/zz,
  * 6..8 advective x/y/z (needs beta_d), 9..11 Kreiss-Oliger x/y/z.
  * upwind selects the advective stencil as in oracle_params. */

@@ -54,6 +54,8 @@ def lib():
                                             ctypes.POINTER(ctypes.c_uint64), P_D, P_D,
                                             ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(OracleParams), ctypes.c_int]
+        L.oracle_constraints.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
+                                         ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]
         L.oracle_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), P_D,
                                    ctypes.c_uint32, ctypes.c_int]
         _lib = L
@@ -89,6 +91,17 @@ def rhs_blocks(u, bl, params, nthreads=1, out=None):
     rc = lib().oracle_bssnrhs_blocks(_ptrs(out), _ptrs(u), nb, offs, pmin, pmax, sz, bf, ctypes.byref(params), nthreads)
     if rc != 0:
         raise RuntimeError(f"oracle_bssnrhs_blocks failed: {rc}")
+    return out
+
+
+def constraints_block(u, sz, pmin, pmax, bflag):
+    """(4, npts): Hamiltonian and momentum constraints at interior points."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros((4, u.shape[1]))
+    rc = lib().oracle_constraints(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin),
+                                  (ctypes.c_double * 3)(*pmax), (ctypes.c_uint32 * 3)(*sz), bflag)
+    if rc != 0:
+        raise RuntimeError(f"oracle_constraints failed: {rc}")
     return out
 
 
@@ -161,3 +174,66 @@ def ref_fake_initial_data(variant, sz, pmin, pmax):
                 L.ref_fake_initial_data(pmin[0] + i * h[0], pmin[1] + j * h[1], pmin[2] + k * h[2], v)
                 u[:, i + sz[0] * (j + sz[1] * k)] = np.frombuffer(v, dtype=np.float64)
     return u
+
+
+# ----------------------------------------------------------------- RK4 + ghosts
+def ghost_fill_uniform(u, bl, nb):
+    """Same-level ghost refresh of an nb^3 uniform tiling (blocks in z-y-x
+    order, equal sizes): every ghost point inside the domain takes the value of
+    the interior point of the block that owns that lattice position.  Written
+    independently of the library's neighbour table (global lattice indices)."""
+    import itertools
+    n = bl.sz[0][0]
+    m = n - 6
+    out = u.copy()
+    idx = np.arange(n)
+    for b in range(len(bl)):
+        bz, rem = divmod(b, nb * nb)
+        by, bx = divmod(rem, nb)
+        blk = np.array([bx, by, bz])
+        o = bl.offsets[b]
+        view = out[:, o:o + n ** 3].reshape(24, n, n, n)  # [f][k][j][i]
+        for i, j, k in itertools.product(range(n), repeat=3):
+            if 3 <= i < n - 3 and 3 <= j < n - 3 and 3 <= k < n - 3:
+                continue
+            g = blk * m + np.array([i, j, k]) - 3          # global lattice index
+            ob = g // m
+            if np.any(ob < 0) or np.any(ob >= nb):
+                continue
+            li = g - ob * m + 3
+            src = ob[0] + nb * (ob[1] + nb * ob[2])
+            so = bl.offsets[src]
+            view[:, k, j, i] = u[:, so + li[0] + n * (li[1] + n * li[2])]
+    return out
+
+
+def rk4_step(u, bl, nb, dt, params, nthreads=1):
+    """Classic RK4 step of the oracle RHS with ghost refresh before every
+    stage; only interior points change (textbook tableau)."""
+    def interior_mask():
+        mk = np.zeros(u.shape[1], dtype=bool)
+        for b in range(len(bl)):
+            n = bl.sz[b][0]
+            v = np.zeros((n, n, n), dtype=bool)
+            v[3:-3, 3:-3, 3:-3] = True
+            mk[bl.offsets[b]:bl.offsets[b] + n ** 3] = v.reshape(-1)
+        return mk
+    mk = interior_mask()
+    un = ghost_fill_uniform(u, bl, nb)
+    k1 = rhs_blocks(un, bl, params, nthreads)
+    acc = un.copy(); s = un.copy()
+    acc[:, mk] = un[:, mk] + (dt / 6) * k1[:, mk]
+    s[:, mk] = un[:, mk] + (dt / 2) * k1[:, mk]
+    s = ghost_fill_uniform(s, bl, nb)
+    k2 = rhs_blocks(s, bl, params, nthreads)
+    acc[:, mk] = acc[:, mk] + (dt / 3) * k2[:, mk]
+    s2 = s.copy(); s2[:, mk] = un[:, mk] + (dt / 2) * k2[:, mk]
+    s2 = ghost_fill_uniform(s2, bl, nb)
+    k3 = rhs_blocks(s2, bl, params, nthreads)
+    acc[:, mk] = acc[:, mk] + (dt / 3) * k3[:, mk]
+    s3 = s2.copy(); s3[:, mk] = un[:, mk] + dt * k3[:, mk]
+    s3 = ghost_fill_uniform(s3, bl, nb)
+    k4 = rhs_blocks(s3, bl, params, nthreads)
+    new = un.copy()
+    new[:, mk] = acc[:, mk] + (dt / 6) * k4[:, mk]
+    return new

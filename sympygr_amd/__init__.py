@@ -89,6 +89,17 @@ _SIGS = {
                                         ctypes.c_int]),
     "sgr_rhs_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "sgr_constraints": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_ghost_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_rk4_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(P_D),
+                                     ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_rk4_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(P_D), ctypes.POINTER(P_D),
+                                    ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_rk4_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(P_D),
+                                            ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.POINTER(P_D),
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "sgr_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sgr_graph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "sgr_check_finite": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     "sgr_version": (ctypes.c_char_p, []),
@@ -191,6 +202,36 @@ class Plan:
         ev = (ctypes.c_void_p * 4)(*events)
         check(lib().sgr_rhs_events(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s), ev))
 
+    @staticmethod
+    def _stream(t, stream):
+        import torch
+        return stream if stream is not None else torch.cuda.current_stream(t.device).cuda_stream
+
+    def constraints(self, u, out4, stream=None):
+        """Hamiltonian (out4[0]) and momentum (out4[1:4]) constraints at the
+        interior points; out4 is a (4, total) float64 tensor."""
+        if out4.dim() != 2 or out4.shape[0] != 4 or not out4.is_contiguous():
+            raise ValueError("out4 must be a contiguous (4, total) tensor")
+        base, st = out4.data_ptr(), out4.stride(0) * out4.element_size()
+        po = (P_D * 4)(*[ctypes.cast(base + c * st, P_D) for c in range(4)])
+        check(lib().sgr_constraints(self._h, _field_ptrs(u), po, ctypes.c_void_p(self._stream(u, stream))))
+
+    def ghost_fill(self, u, stream=None):
+        """Same-level ghost refresh of u (in place)."""
+        check(lib().sgr_ghost_fill(self._h, _field_ptrs(u), ctypes.c_void_p(self._stream(u, stream))))
+
+    def rk4_stage(self, stage, dt, u_n, u_stage, acc, u_next, stream=None):
+        check(lib().sgr_rk4_stage(self._h, stage, dt, _field_ptrs(u_n), _field_ptrs(u_stage), _field_ptrs(acc),
+                                  _field_ptrs(u_next), ctypes.c_void_p(self._stream(u_n, stream))))
+
+    def rk4_step(self, dt, u, s_a, s_b, acc, stream=None):
+        """One RK4 step of u in place (ghost refresh + 4 fused stages)."""
+        check(lib().sgr_rk4_step(self._h, dt, _field_ptrs(u), _field_ptrs(s_a), _field_ptrs(s_b),
+                                 _field_ptrs(acc), ctypes.c_void_p(self._stream(u, stream))))
+
+    def rk4_graph(self, dt, u, s_a, s_b, acc):
+        return Graph(self, dt, u, s_a, s_b, acc)
+
     def check_finite(self, u, stream=None):
         import torch
         s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
@@ -221,3 +262,29 @@ def bssnrhs(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
     check(lib().sgr_bssnrhs_host(po, pi, offset, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
                                  (ctypes.c_uint32 * 3)(*sz), bflag, ctypes.byref(prm), device))
     return out
+
+
+class Graph:
+    """A captured RK4 step (HIP graph) of one plan and fixed buffers."""
+
+    def __init__(self, plan, dt, u, s_a, s_b, acc):
+        self._g = ctypes.c_void_p()
+        self._keep = (plan, u, s_a, s_b, acc)
+        check(lib().sgr_rk4_graph_create(plan._h, dt, _field_ptrs(u), _field_ptrs(s_a), _field_ptrs(s_b),
+                                         _field_ptrs(acc), ctypes.byref(self._g)))
+
+    def launch(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().sgr_graph_launch(self._g, ctypes.c_void_p(s)))
+
+    def close(self):
+        if self._g:
+            lib().sgr_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

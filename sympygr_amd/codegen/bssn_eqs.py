@@ -291,3 +291,44 @@ def bssn_rhs_stages() -> Stages:
     for n in RHS_NAMES:
         S.outputs.append((name_map.get(n, n), out[n]))
     return S
+
+
+def constraint_stages() -> Stages:
+    """Hamiltonian and momentum constraints, GR/constraints.py:161-173, with the
+    Ricci tensor of GR/dendro.py:468-527 (the same staged pieces as the RHS):
+
+        H   = chi igt^{jk} R_jk - A_ij A^ij + 2/3 K^2
+        M_i = igt^{jk} (d_k A_ij - C2[m,k,i] A_jm) - Gt^j A_ij
+              - 3/2 igt^{jk} A_ki d_j chi / chi - 2/3 d_i K
+
+    Outputs are named ham, mom0..2 and stored with BSSN_OUT(0..3, v)."""
+    R = bssn_rhs_stages()
+    keep = {"idet", "igt", "inv_chi", "C1", "C2", "igt_dchi", "CalGt", "C1u", "Rt", "Rphi_tr", "Rphi",
+            "Amix", "AtU"}
+    S = Stages()
+    for name, pairs in R.stages:
+        if name in keep:
+            S.stage(name, pairs)
+    I = Inputs()
+    u = I.u
+    d1 = I.d1
+    chi, K = u["chi"], u["K"]
+    Gt = [u["Gt0"], u["Gt1"], u["Gt2"]]
+    Atn = [[f"At{sym_slot(i, j)}" for j in E3] for i in E3]
+    At = [[u[Atn[i][j]] for j in E3] for i in E3]
+    igt = _sym3("igt")
+    C2 = [[[sp.Symbol(f"C2_{i}_{min(j, k)}{max(j, k)}") for k in E3] for j in E3] for i in E3]
+    Rt = _sym3("Rt")
+    Rphi = _sym3("Rphi")
+    AtU = _sym3("AtU")
+    inv_chi = sp.Symbol("inv_chi")
+    ham = chi * sum(igt[j][k] * (Rt[j][k] + Rphi[j][k]) for (j, k) in E33) \
+        - sum(At[i][j] * AtU[i][j] for (i, j) in E33) + sp.Rational(2, 3) * K * K
+    S.outputs.append(("ham", ham))
+    for i in E3:
+        m = sum(igt[j][k] * (d1(Atn[i][j], k) - sum(C2[mm][k][i] * At[j][mm] for mm in E3)) for (j, k) in E33)
+        m -= sum(Gt[j] * At[i][j] for j in E3)
+        m -= sp.Rational(3, 2) * sum(igt[j][k] * At[k][i] * d1("chi", j) * inv_chi for (j, k) in E33)
+        m -= sp.Rational(2, 3) * d1("K", i)
+        S.outputs.append((f"mom{i}", m))
+    return S

@@ -6,7 +6,9 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <string>
 #include <vector>
@@ -132,7 +134,79 @@ struct sgr_plan {
     uint64_t ws_bytes = 0;
     uint64_t interior = 0;
     uint32_t nslots = 0;
+    // same-level ghost refresh table
+    std::vector<GhostCopy> hghost;
+    GhostCopy *d_ghost = nullptr;
+    bool ghost_complete = true;
+    std::string ghost_msg;
 };
+
+// Same-level neighbour copies for every ghost box of every block.  Blocks are
+// matched on integer lattice positions of their interior origins (in units of
+// their own spacing) and must share sizes and spacings; a ghost box with no
+// neighbour is skipped when it lies beyond a flagged (domain-boundary) face,
+// otherwise the table is marked incomplete (level boundary: needs
+// prolongation, not provided).
+static void build_ghost_table(sgr_plan *p, const sgr_block *blocks, uint32_t nb)
+{
+    if (nb == 0) return;
+    double h0[3], o0[3];
+    for (int d = 0; d < 3; d++) {
+        h0[d] = (blocks[0].pmax[d] - blocks[0].pmin[d]) / (blocks[0].sz[d] - 1);
+        o0[d] = blocks[0].pmin[d];
+    }
+    std::map<std::tuple<long long, long long, long long, long long>, uint32_t> at;
+    auto key = [&](const sgr_block &b, int dx, int dy, int dz) {
+        long long k[3];
+        const int dir[3] = {dx, dy, dz};
+        for (int d = 0; d < 3; d++) {
+            const double h = (b.pmax[d] - b.pmin[d]) / (b.sz[d] - 1);
+            const double o = b.pmin[d] + dir[d] * (double)(b.sz[d] - 6) * h;
+            k[d] = llround((o - o0[d]) / h0[d] * 16.0);  // 1/16-cell lattice of block 0
+        }
+        const double h = (b.pmax[0] - b.pmin[0]) / (b.sz[0] - 1);
+        return std::make_tuple(k[0], k[1], k[2], llround(h / h0[0] * 1024.0));
+    };
+    for (uint32_t b = 0; b < nb; b++) at[key(blocks[b], 0, 0, 0)] = b;
+    for (uint32_t b = 0; b < nb; b++) {
+        const sgr_block &B = blocks[b];
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    if (!dx && !dy && !dz) continue;
+                    const int dir[3] = {dx, dy, dz};
+                    auto it = at.find(key(B, dx, dy, dz));
+                    bool ok = it != at.end();
+                    if (ok) {
+                        const sgr_block &N = blocks[it->second];
+                        ok = N.sz[0] == B.sz[0] && N.sz[1] == B.sz[1] && N.sz[2] == B.sz[2];
+                    }
+                    if (!ok) {
+                        bool outside = false;
+                        for (int d = 0; d < 3; d++)
+                            if (dir[d] && (B.bflag & (1u << (2 * d + (dir[d] > 0 ? 1 : 0))))) outside = true;
+                        if (!outside && p->ghost_complete) {
+                            p->ghost_complete = false;
+                            p->ghost_msg = "block " + std::to_string(b) +
+                                           " has a ghost region with no same-level neighbour (level boundary)";
+                        }
+                        continue;
+                    }
+                    GhostCopy g{};
+                    g.dst_off = B.offset;
+                    g.src_off = blocks[it->second].offset;
+                    g.n0 = (int32_t)B.sz[0];
+                    g.n1 = (int32_t)B.sz[1];
+                    for (int d = 0; d < 3; d++) {
+                        const int n = (int)B.sz[d];
+                        g.d[d] = dir[d] < 0 ? 0 : (dir[d] == 0 ? 3 : n - 3);
+                        g.s[d] = dir[d] < 0 ? n - 6 : 3;
+                        g.e[d] = dir[d] == 0 ? n - 6 : 3;
+                    }
+                    p->hghost.push_back(g);
+                }
+    }
+}
 
 static void plan_free(sgr_plan *p)
 {
@@ -145,6 +219,7 @@ static void plan_free(sgr_plan *p)
     hipFree(p->d_ws);
     hipFree(p->d_wsK);
     hipFree(p->d_flag);
+    hipFree(p->d_ghost);
     hipSetDevice(cur);
     delete p;
 }
@@ -263,6 +338,14 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         }
     }
     if ((e = hipMalloc(&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
+    build_ghost_table(p, blocks, nblocks);
+    if (!p->hghost.empty()) {
+        if ((e = hipMalloc(&p->d_ghost, p->hghost.size() * sizeof(GhostCopy))) != hipSuccess)
+            return bail(e, "hipMalloc(ghost table)");
+        if ((e = hipMemcpy(p->d_ghost, p->hghost.data(), p->hghost.size() * sizeof(GhostCopy),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy(ghost table)");
+    }
     hipSetDevice(cur);
     *plan = p;
     return SGR_OK;
@@ -286,7 +369,7 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
 }
 
 static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
-                    hipStream_t stream, hipEvent_t *ev)
+                    hipStream_t stream, hipEvent_t *ev, const RkArgs *rk = nullptr)
 {
     if (!plan || !out || !in) return fail(SGR_E_INVALID, "null argument");
     LaunchArgs a{};
@@ -305,6 +388,12 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
     for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
+    a.rk.stage = -1;
+    if (rk) {
+        if (plan->variant == Variant::Fused)
+            return fail(SGR_E_UNSUPPORTED, "RK4 stages need the two-pass kernels (unset SGR_KERNEL=fused)");
+        a.rk = *rk;
+    }
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
@@ -332,6 +421,155 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
 {
     if (!events) return fail(SGR_E_INVALID, "events is null");
     return rhs_impl(plan, out, in, stream, events);
+}
+
+int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream)
+{
+    if (!plan || !in || !out) return fail(SGR_E_INVALID, "null argument");
+    if (plan->variant == Variant::Fused)
+        return fail(SGR_E_UNSUPPORTED, "constraints need the two-pass kernels (unset SGR_KERNEL=fused)");
+    LaunchArgs a{};
+    OutPtrs co{};
+    for (int f = 0; f < NF; f++) {
+        if (!in[f]) return fail(SGR_E_INVALID, "null field pointer");
+        a.in.p[f] = in[f];
+    }
+    for (int c = 0; c < 4; c++) {
+        if (!out[c]) return fail(SGR_E_INVALID, "null constraint pointer");
+        co.p[c] = out[c];
+    }
+    a.blocks = plan->d_blocks;
+    a.tile_block = plan->d_tile_block;
+    a.ntiles = plan->ntiles;
+    a.ws = plan->d_ws;
+    a.wsK = plan->d_wsK;
+    a.np = plan->np;
+    a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
+    a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    int cur = 0;
+    hipGetDevice(&cur);
+    if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
+    hipError_t e = launch_constraints(a, co, stream);
+    if (cur != plan->device) hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "constraint kernels");
+    return SGR_OK;
+}
+
+int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t stream)
+{
+    if (!plan || !u) return fail(SGR_E_INVALID, "null argument");
+    if (!plan->ghost_complete) return fail(SGR_E_UNSUPPORTED, plan->ghost_msg);
+    OutPtrs o;
+    for (int f = 0; f < NF; f++) {
+        if (!u[f]) return fail(SGR_E_INVALID, "null field pointer");
+        o.p[f] = u[f];
+    }
+    int cur = 0;
+    hipGetDevice(&cur);
+    if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
+    hipError_t e = launch_ghost_fill(o, plan->d_ghost, (uint32_t)plan->hghost.size(), stream);
+    if (cur != plan->device) hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "ghost fill launch");
+    return SGR_OK;
+}
+
+int sgr_rk4_stage(sgr_plan *plan, int stage, double dt, const double *const u_n[SGR_NUM_VARS],
+                  const double *const u_stage[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS],
+                  double *const u_next[SGR_NUM_VARS], hipStream_t stream)
+{
+    if (!plan || !u_n || !u_stage || !acc || !u_next) return fail(SGR_E_INVALID, "null argument");
+    if (stage < 0 || stage > 3) return fail(SGR_E_INVALID, "stage must be 0..3");
+    if (!std::isfinite(dt)) return fail(SGR_E_INVALID, "dt must be finite");
+    RkArgs rk{};
+    rk.stage = stage;
+    const double cn[3] = {dt / 2, dt / 2, dt};
+    const double w[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
+    rk.cn = stage < 3 ? cn[stage] : 0.0;
+    rk.w = w[stage];
+    for (int f = 0; f < NF; f++) {
+        if (!u_n[f] || !acc[f] || !u_next[f]) return fail(SGR_E_INVALID, "null field pointer");
+        rk.u0.p[f] = u_n[f];
+        rk.acc.p[f] = acc[f];
+        rk.next.p[f] = u_next[f];
+    }
+    return rhs_impl(plan, u_next, u_stage, stream, nullptr, &rk);
+}
+
+int sgr_rk4_step(sgr_plan *plan, double dt, double *const u[SGR_NUM_VARS], double *const s_a[SGR_NUM_VARS],
+                 double *const s_b[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS], hipStream_t stream)
+{
+    if (!plan || !u || !s_a || !s_b || !acc) return fail(SGR_E_INVALID, "null argument");
+    int rc;
+    if ((rc = sgr_ghost_fill(plan, u, stream))) return rc;
+    if ((rc = sgr_rk4_stage(plan, 0, dt, u, u, acc, s_a, stream))) return rc;
+    if ((rc = sgr_ghost_fill(plan, s_a, stream))) return rc;
+    if ((rc = sgr_rk4_stage(plan, 1, dt, u, s_a, acc, s_b, stream))) return rc;
+    if ((rc = sgr_ghost_fill(plan, s_b, stream))) return rc;
+    if ((rc = sgr_rk4_stage(plan, 2, dt, u, s_b, acc, s_a, stream))) return rc;
+    if ((rc = sgr_ghost_fill(plan, s_a, stream))) return rc;
+    return sgr_rk4_stage(plan, 3, dt, u, s_a, acc, u, stream);
+}
+
+struct sgr_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+int sgr_rk4_graph_create(sgr_plan *plan, double dt, double *const u[SGR_NUM_VARS], double *const s_a[SGR_NUM_VARS],
+                         double *const s_b[SGR_NUM_VARS], double *const acc[SGR_NUM_VARS], sgr_graph **graph)
+{
+    if (!plan || !graph) return fail(SGR_E_INVALID, "null argument");
+    *graph = nullptr;
+    int cur = 0;
+    hipGetDevice(&cur);
+    HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        hipSetDevice(cur);
+        return hip_fail(e, "hipStreamCreate");
+    }
+    sgr_graph *g = new sgr_graph;
+    g->device = plan->device;
+    int rc = SGR_OK;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) rc = hip_fail(e, "hipStreamBeginCapture");
+    if (rc == SGR_OK) {
+        rc = sgr_rk4_step(plan, dt, u, s_a, s_b, acc, s);
+        e = hipStreamEndCapture(s, &g->graph);
+        if (rc == SGR_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    }
+    if (rc == SGR_OK) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+    hipStreamDestroy(s);
+    hipSetDevice(cur);
+    if (rc != SGR_OK) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *graph = g;
+    return SGR_OK;
+}
+
+int sgr_graph_launch(sgr_graph *graph, hipStream_t stream)
+{
+    if (!graph) return fail(SGR_E_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(graph->exec, stream), "hipGraphLaunch");
+    return SGR_OK;
+}
+
+int sgr_graph_destroy(sgr_graph *graph)
+{
+    if (!graph) return SGR_OK;
+    if (graph->exec) hipGraphExecDestroy(graph->exec);
+    if (graph->graph) hipGraphDestroy(graph->graph);
+    delete graph;
+    return SGR_OK;
 }
 
 int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipStream_t stream)

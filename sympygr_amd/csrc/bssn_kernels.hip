@@ -67,7 +67,7 @@ __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && 
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
-                                                double *__restrict__ wsK, uint64_t SS)
+                                                double *__restrict__ wsK, uint64_t SS, bool all_d1)
 {
     constexpr int NHALO = TH * TH * TH;            // 2744
     constexpr int NPF = (NHALO + 255) / 256;       // 11 prefetch registers per thread
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
 #pragma unroll
             for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
         }
-        const bool need_d1 = !UPWIND || !d1_only_for_faces(f) || face_tile;
+        const bool need_d1 = all_d1 || !UPWIND || !d1_only_for_faces(f) || face_tile;
 #pragma unroll
         for (int p = 0; p < 2; p++) {
             if (!valid[p]) continue;
@@ -230,6 +230,20 @@ __device__ __forceinline__ double ko_add(double v, double sigma, double k)
     return v + sigma * k;
 }
 
+// Classic RK4 stage update of one field at one point (see RkArgs).
+__device__ __forceinline__ void rk_store(const RkArgs &rk, int f, uint64_t pp, double k)
+{
+    SGR_EXACT
+    const double wk = rk.w * k;
+    if (rk.stage == 3) {
+        rk.next.p[f][pp] = rk.acc.p[f][pp] + wk;
+        return;
+    }
+    const double u0 = rk.u0.p[f][pp];
+    rk.acc.p[f][pp] = (rk.stage == 0 ? u0 : rk.acc.p[f][pp]) + wk;
+    rk.next.p[f][pp] = u0 + rk.cn * k;
+}
+
 #define SGR_PW_NAME k_pointwise_g1
 #define SGR_PW_ALG "gen/bssn_algebra_g1.inc"
 #include "pointwise_body.h"
@@ -247,31 +261,108 @@ __device__ __forceinline__ double ko_add(double v, double sigma, double k)
 #undef SGR_PW_ALG
 
 // ------------------------------------------------------------------ launch
-template <typename R, typename W, bool UPWIND>
+template <typename R, typename W, bool UPWIND, bool RK>
 static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
 {
     W *ws = (W *)a.ws;
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
     hipExtLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                          a.blocks, a.tile_block, ws, a.wsK, a.np);
+                          a.blocks, a.tile_block, ws, a.wsK, a.np, false);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const dim3 g(2 * a.ntiles), b(256);
-    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
-    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND>), g, b, 0, s, nullptr, nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
-    hipExtLaunchKernelGGL((k_pointwise_g3<R, W, UPWIND>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params);
+    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
+    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND, RK>), g, b, 0, s, nullptr, nullptr, 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
+    hipExtLaunchKernelGGL((k_pointwise_g3<R, W, UPWIND, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
     return hipGetLastError();
+}
+
+template <typename R, typename W, bool UPWIND>
+static hipError_t launch_rk(const LaunchArgs &a, hipStream_t s)
+{
+    return a.rk.stage >= 0 ? launch_t<R, W, UPWIND, true>(a, s) : launch_t<R, W, UPWIND, false>(a, s);
 }
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s)
 {
     if (a.ntiles == 0) return hipSuccess;
-    if (a.mixed) return a.upwind ? launch_t<float, float, true>(a, s) : launch_t<float, float, false>(a, s);
-    return a.upwind ? launch_t<double, double, true>(a, s) : launch_t<double, double, false>(a, s);
+    if (a.mixed) return a.upwind ? launch_rk<float, float, true>(a, s) : launch_rk<float, float, false>(a, s);
+    return a.upwind ? launch_rk<double, double, true>(a, s) : launch_rk<double, double, false>(a, s);
+}
+
+// Hamiltonian and momentum constraints (GR/constraints.py:161-173) at every
+// interior point, in double, from the derivative workspace.
+template <typename W, bool UPWIND>
+__global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout, const BlockDesc *__restrict__ blocks,
+                                                     const uint32_t *__restrict__ tile_block, const W *__restrict__ ws,
+                                                     uint64_t SS)
+{
+    const uint32_t tile = blockIdx.x >> 1;
+    const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
+    const BlockDesc &B = blocks[tile_block[tile]];
+    const TileCoord tc = tile_coord(B, tile);
+    const int i = tc.i0 + (l & 7), j = tc.j0 + ((l >> 3) & 7), k = tc.k0 + (l >> 6);
+    const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+    if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
+    const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
+    const W *__restrict__ wsb = ws + (uint64_t)tile * TILE_PTS + l;
+    typedef double R;
+#define BSSN_U(F_) ((R)in.p[F_][pp])
+#define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
+#define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
+#define BSSN_OUT(C_, V_) cout.p[C_][pp] = (V_)
+#define BSSN_BARRIER()
+#include "gen/bssn_constraints.inc"
+#undef BSSN_U
+#undef BSSN_D1
+#undef BSSN_D2
+#undef BSSN_OUT
+#undef BSSN_BARRIER
+}
+
+template <typename W, bool UPWIND>
+static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
+{
+    W *ws = (W *)a.ws;
+    hipLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws, a.wsK,
+                       a.np, true);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_constraints<W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
+                       a.tile_block, (const W *)ws, a.np);
+    return hipGetLastError();
+}
+
+hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
+{
+    if (a.ntiles == 0) return hipSuccess;
+    if (a.mixed) return a.upwind ? launch_constraints_t<float, true>(a, cout, s) : launch_constraints_t<float, false>(a, cout, s);
+    return a.upwind ? launch_constraints_t<double, true>(a, cout, s) : launch_constraints_t<double, false>(a, cout, s);
+}
+
+// One workgroup per copy box; threads walk the box, all 24 fields.
+__global__ __launch_bounds__(256) void k_ghost_fill(OutPtrs u, const GhostCopy *__restrict__ copies)
+{
+    const GhostCopy c = copies[blockIdx.x];
+    const int npts = c.e[0] * c.e[1] * c.e[2];
+    for (int q = threadIdx.x; q < npts; q += 256) {
+        const int x = q % c.e[0], y = (q / c.e[0]) % c.e[1], z = q / (c.e[0] * c.e[1]);
+        const uint64_t dp = c.dst_off + (c.d[0] + x) + (uint64_t)c.n0 * ((c.d[1] + y) + (uint64_t)c.n1 * (c.d[2] + z));
+        const uint64_t sp = c.src_off + (c.s[0] + x) + (uint64_t)c.n0 * ((c.s[1] + y) + (uint64_t)c.n1 * (c.s[2] + z));
+#pragma unroll 4
+        for (int f = 0; f < NF; f++) u.p[f][dp] = u.p[f][sp];
+    }
+}
+
+hipError_t launch_ghost_fill(OutPtrs u, const GhostCopy *copies, uint32_t ncopies, hipStream_t s)
+{
+    if (ncopies == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost_fill, dim3(ncopies), dim3(256), 0, s, u, copies);
+    return hipGetLastError();
 }
 
 }  // namespace sgr

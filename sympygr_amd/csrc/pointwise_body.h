@@ -4,11 +4,11 @@
 // runs the group's part of the generated algebra in R (double or float),
 // applies the Sommerfeld condition on flagged faces and adds sigma * KO in
 // double.
-template <typename R, typename W, bool UPWIND>
+template <typename R, typename W, bool UPWIND, bool RK>
 __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
                                                    const uint32_t *__restrict__ tile_block,
                                                    const W *__restrict__ ws, const double *__restrict__ wsK,
-                                                   uint64_t SS, Params P)
+                                                   uint64_t SS, Params P, RkArgs rk)
 {
     const uint32_t tile = blockIdx.x >> 1;
     const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
@@ -49,14 +49,17 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
 #define BSSN_AD(F_, D_) ((R)wsb[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * SS])
 #define BSSN_OUT(F_, V_)                                                                                      \
     do {                                                                                                    \
-        double val_ = (double)(V_);                                                                          \
+        double val_ = (double)(V_);                                                                         \
         if (face)                                                                                           \
-            val_ = sommerfeld(x, y, z, inv_r, (double)wsb[slot_d1(F_, 0) * SS],                          \
-                              (double)wsb[slot_d1(F_, 1) * SS], (double)wsb[slot_d1(F_, 2) * SS],     \
-                              in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                      \
-        out.p[F_][pp] = ko_add(val_, sigma, kb[(F_) * SS]);                                   \
+            val_ = sommerfeld(x, y, z, inv_r, (double)wsb[slot_d1(F_, 0) * SS],                             \
+                              (double)wsb[slot_d1(F_, 1) * SS], (double)wsb[slot_d1(F_, 2) * SS],           \
+                              in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                   \
+        val_ = ko_add(val_, sigma, kb[(F_) * SS]);                                                          \
+        if (RK)                                                                                             \
+            rk_store(rk, F_, pp, val_);                                                                     \
+        else                                                                                                \
+            out.p[F_][pp] = val_;                                                                           \
     } while (0)
-#define BSSN_BARRIER() SGR_ALGEBRA_BARRIER
 #include SGR_PW_ALG
 #undef BSSN_BARRIER
 #undef BSSN_U

@@ -300,9 +300,38 @@ def octree_blocks(domain=16.0, target=3000, sizes=(16, 24, 32, 48), centres=((2.
     return bl
 
 
-def fill_blocks(bl: BlockList, backend="numpy", device="cpu", kind="bl", dtype=None):
+def perturbed_flat_fill(xp, x, y, z, out, seed=1, amp=1e-3, lo=-1.0, L=2.0):
+    """Flat background plus amp * 8 seeded Fourier modes per field evaluated
+    at global coordinates over a cube [lo, lo+L]^3 (consistent across
+    blocks), metric renormalised to det 1 and At made trace-free."""
+    for f in range(NFIELDS):
+        acc = 0.0 * x
+        for (kx, ky, kz, ph) in fourier_modes(seed, f):
+            acc = acc + xp.sin(2.0 * math.pi * (kx * (x - lo) + ky * (y - lo) + kz * (z - lo)) / L + ph)
+        out[f] = amp * acc
+    out[ALPHA] = out[ALPHA] + 1.0
+    out[CHI] = out[CHI] + 1.0
+    for s in (0, 3, 5):
+        out[GT[s]] = out[GT[s]] + 1.0
+    g = [out[i] for i in GT]
+    det = g[0] * (g[3] * g[5] - g[4] ** 2) - g[1] * (g[1] * g[5] - g[2] * g[4]) + g[2] * (g[1] * g[4] - g[2] * g[3])
+    sc = 1.0 / xp.cbrt(det)
+    for i in GT:
+        out[i] = out[i] * sc
+    g = [out[i] for i in GT]
+    det = g[0] * (g[3] * g[5] - g[4] ** 2) - g[1] * (g[1] * g[5] - g[2] * g[4]) + g[2] * (g[1] * g[4] - g[2] * g[3])
+    inv = [(g[3] * g[5] - g[4] ** 2) / det, (g[2] * g[4] - g[1] * g[5]) / det, (g[1] * g[4] - g[2] * g[3]) / det,
+           (g[0] * g[5] - g[2] ** 2) / det, (g[1] * g[2] - g[0] * g[4]) / det, (g[0] * g[3] - g[1] ** 2) / det]
+    w = [1, 2, 2, 1, 2, 1]
+    tr = sum(w[s] * inv[s] * out[AT[s]] for s in range(6))
+    for s in range(6):
+        out[AT[s]] = out[AT[s]] - g[s] * tr / 3.0
+
+
+def fill_blocks(bl: BlockList, backend="numpy", device="cpu", kind="bl", **kw):
     """Allocate a (24, total) float64 array and fill every padded block with
-    the analytic data (ghosts included)."""
+    the analytic data (ghosts included): kind "bl" (Brill-Lindquist) or
+    "perturbed" (perturbed_flat_fill, keywords seed, amp, lo, L)."""
     xp = _xp(backend, device)
     u = xp.zeros((NFIELDS, bl.total))
     for b in range(len(bl)):
@@ -312,6 +341,8 @@ def fill_blocks(bl: BlockList, backend="numpy", device="cpu", kind="bl", dtype=N
         vals = [None] * NFIELDS
         if kind == "bl":
             brill_lindquist_fill(xp, x, y, z, vals)
+        elif kind == "perturbed":
+            perturbed_flat_fill(xp, x, y, z, vals, **kw)
         else:
             raise ValueError(kind)
         o = bl.offsets[b]

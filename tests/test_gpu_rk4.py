@@ -1,0 +1,124 @@
+"""GPU tests of the same-level ghost refresh, the fused RK4 stages and the
+captured RK4-step graph, against the oracle RK4 (oracle RHS, textbook
+tableau, independent ghost fill)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NB, N = 2, 16
+DT = 0.25 * (2.0 / (NB * N))
+
+
+def _setup():
+    import torch
+    from sympygr_amd import data
+    bl = data.uniform_tiling(nb=NB, n=N, lo=-1.0, hi=1.0, stagger=0.5, faces=True)
+    u = data.fill_blocks(bl, kind="perturbed", seed=3, amp=1e-2, lo=-1.0, L=2.0)
+    return torch, bl, u
+
+
+def _interior_mask(bl):
+    mk = np.zeros(bl.total, dtype=bool)
+    for b in range(len(bl)):
+        n = bl.sz[b][0]
+        v = np.zeros((n, n, n), dtype=bool)
+        v[3:-3, 3:-3, 3:-3] = True
+        mk[bl.offsets[b]:bl.offsets[b] + n ** 3] = v.reshape(-1)
+    return mk
+
+
+def _err(a, b, mk):
+    a, b = a[:, mk], b[:, mk]
+    return max(np.abs(a[f] - b[f]).max() / max(np.abs(b[f]).max(), 1e-300) for f in range(24))
+
+
+def test_ghost_fill_matches_oracle():
+    import pyoracle as po
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    rng = np.random.default_rng(0)
+    noisy = u + 0.0
+    mk = _interior_mask(bl)
+    noisy[:, ~mk] = rng.standard_normal((24, (~mk).sum()))  # scramble all ghosts
+    plan = sgr.Plan(bl, sgr.params(adv="upwind"))
+    du = torch.from_numpy(noisy).cuda()
+    plan.ghost_fill(du)
+    torch.cuda.synchronize()
+    ref = po.ghost_fill_uniform(noisy, bl, NB)
+    np.testing.assert_array_equal(du.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("adv", ["upwind", "centred"])
+def test_rk4_step_matches_oracle(adv):
+    import pyoracle as po
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params(adv=adv, ko_sigma=0.1))
+    du = torch.from_numpy(u).cuda()
+    sa, sb, acc = torch.zeros_like(du), torch.zeros_like(du), torch.zeros_like(du)
+    P = po.make_params(upwind=adv == "upwind", ko_sigma=0.1)
+    ref = u
+    mk = _interior_mask(bl)
+    for step in range(2):
+        plan.rk4_step(DT, du, sa, sb, acc)
+        torch.cuda.synchronize()
+        ref = po.rk4_step(ref, bl, NB, DT, P, nthreads=4)
+        e = _err(du.cpu().numpy(), ref, mk)
+        print(f"{adv} step {step} err {e:.2e}", flush=True)
+        assert e <= 1e-12
+
+
+def test_rk4_graph_replays_match_eager():
+    """The captured step, replayed three times with eager work and syncs in
+    between, equals three eager steps bit for bit."""
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", ko_sigma=0.1))
+    de = torch.from_numpy(u).cuda()
+    dg = de.clone()
+    bufs_e = [torch.zeros_like(de) for _ in range(3)]
+    bufs_g = [torch.zeros_like(de) for _ in range(3)]
+    g = plan.rk4_graph(DT, dg, *bufs_g)
+    for rep in range(3):
+        plan.rk4_step(DT, de, *bufs_e)
+        torch.cuda.synchronize()
+        _ = (de * 2.0).sum().item()  # unrelated eager work between replays
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(dg, de), f"replay {rep} differs"
+    g.close()
+
+
+def test_rk4_stage_rejects_bad_arguments():
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params())
+    du = torch.from_numpy(u).cuda()
+    with pytest.raises(sgr.SgrError):
+        plan.rk4_stage(4, DT, du, du, du, du)
+    with pytest.raises(sgr.SgrError):
+        plan.rk4_stage(0, float("nan"), du, du, du, du)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_constraints_match_oracle(precision):
+    """GPU constraint kernel vs the oracle on perturbed multi-block data (the
+    mixed plan stages derivatives in FP32, hence its looser bound)."""
+    import pyoracle as po
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=precision))
+    du = torch.from_numpy(u).cuda()
+    c = torch.zeros((4, bl.total), dtype=torch.float64, device="cuda")
+    plan.constraints(du, c)
+    torch.cuda.synchronize()
+    got = c.cpu().numpy()
+    for b in range(len(bl)):
+        o, s = bl.offsets[b], bl.sz[b]
+        n = s[0] * s[1] * s[2]
+        ref = po.constraints_block(u[:, o:o + n], s, bl.pmin[b], bl.pmax[b], bl.bflag[b])
+        g = got[:, o:o + n].reshape(4, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3].reshape(4, -1)
+        r = ref.reshape(4, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3].reshape(4, -1)
+        e = max(np.abs(g[q] - r[q]).max() / np.abs(r[q]).max() for q in range(4))
+        assert e <= (1e-11 if precision == "fp64" else 1e-5), (b, e)
