@@ -161,6 +161,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-octree", action="store_true",
+                    help="skip the batched uneven-octree measurement (config 2)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference point for HBM counters)")
     args = ap.parse_args()
@@ -223,6 +225,30 @@ def main():
         "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                         "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
     }
+    octree = None
+    if not args.no_octree:
+        del u, out, ref_out
+        torch.cuda.empty_cache()
+        obl = data.octree_blocks()
+        log(f"octree: {len(obl)} uneven blocks, {obl.interior_points()} interior points")
+        ou = data.fill_blocks(obl, backend="torch", device="cuda")
+        oout = torch.zeros_like(ou)
+        opts = obl.interior_points()
+        octree = {"workload": "3032 octree leaves over [-16,16]^3 refined toward the punctures, interior "
+                              "16/24/32/48 (+3 ghost), Brill-Lindquist, upwind, one RHS (RK4 stage) per step",
+                  "blocks": len(obl), "interior_points": opts}
+        for prec in ("fp64", "mixed"):
+            op = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+            w, _, _ = timed_steps(op, oout, ou, max(2, args.steps // 4), 1, hev)
+            octree[prec] = {"value": opts * max(2, args.steps // 4) / w / 1e6, "chunks": op.info().chunks,
+                            "ms_per_step": 1e3 * w / max(2, args.steps // 4)}
+            log(f"octree {prec}: {octree[prec]['value']:.1f} Mpts/s in {op.info().chunks} workspace chunks")
+            del op
+        octree["mixed_speedup_vs_fp64"] = octree["mixed"]["value"] / octree["fp64"]["value"]
+        del ou, oout
+        torch.cuda.empty_cache()
+        u = data.fill_blocks(bl, backend="torch", device="cuda")
+
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -244,7 +270,7 @@ def main():
         "config": {"workload": "512 blocks (8^3 tiling of [-8,8]^3) of 32^3 interior + 3 ghost (38^3), "
                                "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
-        "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed,
+        "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
     }
     print(json.dumps(line), flush=True)
 

@@ -75,6 +75,7 @@ typedef struct {
     uint64_t interior_points;
     uint64_t workspace_bytes;    /* device scratch held by the plan */
     uint32_t derivative_slots;   /* derivative values staged per point */
+    uint32_t chunks;             /* launches per pass (workspace budget SGR_WS_GIB) */
 } sgr_plan_info;
 
 /* Replaces the per-block setup of the reference bssnRHS block loop
@@ -95,9 +96,10 @@ int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const
             hipStream_t stream);
 
 /* Same as sgr_rhs, and records events[0..1] around the derivative kernel and
- * events[2..3] around the pointwise kernel on `stream` (hipEventElapsedTime
- * then gives each kernel's duration).  No reference counterpart: it exists
- * for measurement. */
+ * events[2..3] around the pointwise kernels on `stream` (hipEventElapsedTime
+ * then gives each pass's duration).  For a plan that runs in several
+ * workspace chunks, events[0..1] bracket the whole call and events[2..3] are
+ * recorded at its end.  No reference counterpart: it exists for measurement. */
 int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
                    hipStream_t stream, hipEvent_t events[4]);
 

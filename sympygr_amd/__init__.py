@@ -62,6 +62,7 @@ class SgrPlanInfo(ctypes.Structure):
         ("interior_points", ctypes.c_uint64),
         ("workspace_bytes", ctypes.c_uint64),
         ("derivative_slots", ctypes.c_uint32),
+        ("chunks", ctypes.c_uint32),
     ]
 
 

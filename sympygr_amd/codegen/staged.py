@@ -44,18 +44,42 @@ class StagedEmitted:
         self.live_profile = live_profile
 
 
-def emit_staged(S, field_order=None, indent="    ") -> StagedEmitted:
-    order = field_order or DEFAULT_FIELD_ORDER
+def emit_staged(S, field_order=None, indent="    ", outputs=None) -> StagedEmitted:
+    """outputs: indices of S.outputs to emit (default all).  Only fields whose
+    derivatives the selected outputs need, or which are selected outputs
+    themselves (Kreiss-Oliger, boundary value), are staged."""
+    order = list(field_order or DEFAULT_FIELD_ORDER)
     assert sorted(order) == sorted(FIELDS)
-    stage_of_field = {f: i for i, f in enumerate(order)}
 
     names, exprs = [], []
     for _, pairs in S.stages:
         for s, e in pairs:
             names.append(s)
             exprs.append(e)
-    out_names = [n for n, _ in S.outputs]
-    out_exprs = [e for _, e in S.outputs]
+    sel = list(range(len(S.outputs))) if outputs is None else list(outputs)
+    out_names = [S.outputs[i][0] for i in sel]
+    out_exprs = [S.outputs[i][1] for i in sel]
+    out_fields = [FIELDS[i] for i in sel]
+    # fields that must be staged: those with a derivative input reachable
+    # from the selected outputs, plus the output fields themselves
+    defs0 = dict(zip(names, exprs))
+    need = set(out_fields)
+    seen_sym = set()
+    stack = list(out_exprs)
+    while stack:
+        e = stack.pop()
+        for x in e.free_symbols:
+            if x in seen_sym:
+                continue
+            seen_sym.add(x)
+            if x in defs0:
+                stack.append(defs0[x])
+            else:
+                c = classify_input(x.name)
+                if c is not None and c[0] != "U":
+                    need.add(c[1])
+    order = [f for f in order if f in need]
+    stage_of_field = {f: i for i, f in enumerate(order)}
     repl, reduced = sp.cse(exprs + out_exprs, symbols=sp.numbered_symbols("t_"))
     defs = {}
     for s, e in repl:
@@ -63,6 +87,15 @@ def emit_staged(S, field_order=None, indent="    ") -> StagedEmitted:
     for s, e in zip(names, reduced[: len(names)]):
         defs[s] = e
     out_reduced = reduced[len(names):]
+    # keep only what the selected outputs reach
+    reach, stack = set(), list(out_reduced)
+    while stack:
+        e = stack.pop()
+        for x in e.free_symbols:
+            if x in defs and x not in reach:
+                reach.add(x)
+                stack.append(defs[x])
+    defs = {k: v for k, v in defs.items() if k in reach}
 
     # ready stage of every symbol: field values -1, derivatives of f at stage(f)
     ready = {}
@@ -118,11 +151,11 @@ def emit_staged(S, field_order=None, indent="    ") -> StagedEmitted:
 
     out_ready = [max([rdy(x) for x in e.free_symbols] + [-1]) for e in out_reduced]
     for i, n in enumerate(out_names):
-        f = FIELDS[i]
+        f = out_fields[i]
         assert out_ready[i] >= stage_of_field[f], (n, "output ready before its own field is staged")
 
     # derivative inputs used, grouped by stage
-    deriv_inputs = {s: [] for s in range(len(order))}
+    deriv_inputs = {st: [] for st in range(len(order))}
     seen = set()
     for e in list(defs.values()) + out_reduced:
         for x in e.free_symbols:
@@ -200,7 +233,7 @@ def emit_staged(S, field_order=None, indent="    ") -> StagedEmitted:
                 nonlocal_flops(e)
                 step[0] += 1
                 touch_uses(e)
-                lines.append(f"{indent}BSSN_OUT({i}, {printer.doprint(e)});")
+                lines.append(f"{indent}BSSN_OUT({FIELD_INDEX[out_fields[i]]}, {printer.doprint(e)});")
 
     lines.append(f"{indent}BSSN_FIRST_FIELD({FIELD_INDEX[order[0]]});")
     emit_ready(-1)
@@ -212,7 +245,8 @@ def emit_staged(S, field_order=None, indent="    ") -> StagedEmitted:
             lines.append(f"{indent}const R {x.name} = BSSN_{kind}({fi}, {idx});")
             step[0] += 1
             pos_def[x] = step[0]
-        lines.append(f"{indent}BSSN_FIELD_DONE({fi});")
+        if f in out_fields:
+            lines.append(f"{indent}BSSN_FIELD_DONE({fi});")
         emit_ready(s)
         lines.append(f"{indent}BSSN_STAGE_END({fi}, {s}, {fnext});")
 

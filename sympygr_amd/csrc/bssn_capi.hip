@@ -130,7 +130,8 @@ struct sgr_plan {
     void *d_ws = nullptr;
     double *d_wsK = nullptr;
     int *d_flag = nullptr;
-    uint64_t np = 0;
+    uint64_t np = 0;             // workspace plane stride (points)
+    uint32_t chunk_tiles = 0;    // tiles per launch (workspace capacity)
     uint64_t ws_bytes = 0;
     uint64_t interior = 0;
     uint32_t nslots = 0;
@@ -299,16 +300,25 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         return derr != hipSuccess ? hip_fail(derr, "hipGetDeviceCount") : fail(SGR_E_INVALID, "device index out of range");
     }
     p->ntiles = (uint32_t)tile_block.size();
-    // Plane stride: the point count padded by a few KiB so consecutive slot
-    // planes do not start at the same large power-of-two alignment (which
-    // maps every plane of a wave's reads onto the same HBM channels).
+    // The workspace holds at most `chunk_tiles` tiles (budget SGR_WS_GIB,
+    // default 40 GiB); larger plans run as several launches over tile
+    // ranges.  Plane stride: the chunk's point count padded by a few KiB so
+    // consecutive slot planes do not share one power-of-two alignment.
+    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
     {
+        double gib = 40.0;
+        const char *gv = getenv("SGR_WS_GIB");
+        if (gv) gib = atof(gv);
+        const uint64_t per_tile = (uint64_t)TILE_PTS * ((uint64_t)p->nslots * wsz + NF * sizeof(double));
+        uint64_t cap = (uint64_t)(gib * 1073741824.0) / (per_tile ? per_tile : 1);
+        if (cap < 1) cap = 1;
+        const uint64_t nt = tile_block.size();
+        p->chunk_tiles = (uint32_t)(fusedv ? nt : (nt < cap ? nt : cap));
         uint64_t pad = 2048 + 64;
         const char *pv = getenv("SGR_WS_PAD");
         if (pv) pad = strtoull(pv, nullptr, 10);
-        p->np = ws_base + (ws_base ? pad : 0);
+        p->np = fusedv ? 0 : (uint64_t)p->chunk_tiles * TILE_PTS + (nt ? pad : 0);
     }
-    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
     p->ws_bytes = fusedv ? 0 : p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
 
     int cur = 0;
@@ -365,6 +375,7 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
     info->interior_points = plan->interior;
     info->workspace_bytes = plan->ws_bytes;
     info->derivative_slots = plan->nslots;
+    info->chunks = plan->chunk_tiles ? (plan->ntiles + plan->chunk_tiles - 1) / plan->chunk_tiles : 0;
     return SGR_OK;
 }
 
@@ -397,14 +408,27 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (plan->variant == Variant::Fused) {
         e = launch_fused(a, stream);
         // one kernel: the second event pair brackets nothing
         if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
         if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
-    } else {
+    } else if (plan->ntiles <= plan->chunk_tiles) {
         e = launch_rhs(a, stream);
+    } else {
+        // several workspace chunks: events[0..1] bracket the whole call,
+        // events[2..3] are recorded at its end
+        if (ev) e = hipEventRecord(ev[0], stream);
+        for (int q = 0; q < 4; q++) a.ev[q] = nullptr;
+        for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->chunk_tiles) {
+            a.tile0 = t0;
+            a.ntiles = plan->ntiles - t0 < plan->chunk_tiles ? plan->ntiles - t0 : plan->chunk_tiles;
+            e = launch_rhs(a, stream);
+        }
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
     }
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
@@ -449,7 +473,12 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = launch_constraints(a, co, stream);
+    hipError_t e = hipSuccess;
+    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->chunk_tiles) {
+        a.tile0 = t0;
+        a.ntiles = plan->ntiles - t0 < plan->chunk_tiles ? plan->ntiles - t0 : plan->chunk_tiles;
+        e = launch_constraints(a, co, stream);
+    }
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "constraint kernels");
     return SGR_OK;

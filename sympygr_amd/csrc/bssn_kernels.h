@@ -23,7 +23,8 @@ struct LaunchArgs {
     OutPtrs out;
     const BlockDesc *blocks;     // device
     const uint32_t *tile_block;  // device, block id of every tile
-    uint32_t ntiles;
+    uint32_t ntiles;             // tiles of this launch (one workspace chunk)
+    uint32_t tile0;              // first tile of this launch
     void *ws;                    // derivative workspace [slot][np padded to the stride] (double or float)
     double *wsK;                 // KO sums [24][stride]
     uint64_t np;                 // plane stride in points (>= ntiles * 512)

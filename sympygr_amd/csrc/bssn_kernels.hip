@@ -67,7 +67,7 @@ __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && 
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
-                                                double *__restrict__ wsK, uint64_t SS, bool all_d1)
+                                                double *__restrict__ wsK, uint64_t SS, bool all_d1, uint32_t tile0)
 {
     constexpr int NHALO = TH * TH * TH;            // 2744
     constexpr int NPF = (NHALO + 255) / 256;       // 11 prefetch registers per thread
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
 
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = tile0 + blockIdx.x;  // chunk-relative launch
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     bool bpos[2][3];
     // slot-major workspace with a padded plane stride SS (points); this tile's
     // 512 points are contiguous within every plane
-    W *__restrict__ wsb = ws + (uint64_t)tile * TILE_PTS;
-    double *__restrict__ kb = wsK + (uint64_t)tile * TILE_PTS;
+    W *__restrict__ wsb = ws + (uint64_t)blockIdx.x * TILE_PTS;
+    double *__restrict__ kb = wsK + (uint64_t)blockIdx.x * TILE_PTS;
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         pk[p] = tc.k0 + lz + 4 * p;
@@ -268,16 +268,16 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
     hipExtLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                          a.blocks, a.tile_block, ws, a.wsK, a.np, false);
+                          a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const dim3 g(2 * a.ntiles), b(256);
     hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
     hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND, RK>), g, b, 0, s, nullptr, nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
     hipExtLaunchKernelGGL((k_pointwise_g3<R, W, UPWIND, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk);
+                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
     return hipGetLastError();
 }
 
@@ -299,9 +299,9 @@ hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s)
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, const W *__restrict__ ws,
-                                                     uint64_t SS)
+                                                     uint64_t SS, uint32_t tile0)
 {
-    const uint32_t tile = blockIdx.x >> 1;
+    const uint32_t tile = tile0 + (blockIdx.x >> 1);
     const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
-    const W *__restrict__ wsb = ws + (uint64_t)tile * TILE_PTS + l;
+    const W *__restrict__ wsb = ws + (uint64_t)(blockIdx.x >> 1) * TILE_PTS + l;
     typedef double R;
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
@@ -329,11 +329,11 @@ static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStr
 {
     W *ws = (W *)a.ws;
     hipLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws, a.wsK,
-                       a.np, true);
+                       a.np, true, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_constraints<W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
-                       a.tile_block, (const W *)ws, a.np);
+                       a.tile_block, (const W *)ws, a.np, a.tile0);
     return hipGetLastError();
 }
 

@@ -234,54 +234,38 @@ def uniform_tiling(nb=8, n=32, lo=-8.0, hi=8.0, stagger=0.5, faces=True):
     return bl
 
 
-def octree_blocks(domain=16.0, target=3000, sizes=(16, 24, 32, 48), centres=((2.0, 0.0, 0.0), (-2.0, 0.0, 0.0)),
-                  refine_r=2.0, stagger=0.5, max_level=8):
-    """Deterministic uneven block set over [-domain, domain]^3.
+def octree_blocks(domain=16.0, sizes=(16, 24, 32, 48), centres=((2.0, 0.0, 0.0), (-2.0, 0.0, 0.0)), c=1.5,
+                  lmin=2, lmax=7, stagger=0.5):
+    """Deterministic uneven block set over [-domain, domain]^3 refined toward
+    the punctures: an octant of side s is split while it is coarser than
+    level lmin, or (up to level lmax) while its distance to the nearest
+    puncture is below c*s.  With the defaults: 3032 leaves, finest side 0.25.
+    Each leaf's interior size comes from `sizes`, chosen by SplitMix64 of its
+    id; dx = side / size; nodes staggered by `stagger` cells.  bflag marks the
+    faces on the domain boundary."""
+    leaves = []
 
-    Octants are refined while they intersect a ball of radius
-    ``refine_r * 2^(2-level)``... around a puncture or are larger than the
-    coarsest allowed size, until about ``target`` leaves exist.  Each leaf gets
-    an interior size from ``sizes`` chosen from its id (deterministic), and
-    dx = side / size.  bflag marks leaves that touch the outer boundary."""
-    leaves = [(0, (-domain, -domain, -domain), 2.0 * domain)]
-
-    def dist_to_puncture(lo, side):
+    def dist(lo, side):
         best = 1e30
-        for c in centres:
+        for p in centres:
             d2 = 0.0
             for k in range(3):
-                q = min(max(c[k], lo[k]), lo[k] + side)
-                d2 += (q - c[k]) ** 2
+                q = min(max(p[k], lo[k]), lo[k] + side)
+                d2 += (q - p[k]) ** 2
             best = min(best, math.sqrt(d2))
         return best
 
-    level = 0
-    while len(leaves) < target and level < max_level:
-        # refine the leaves closest to the punctures first
-        order = sorted(range(len(leaves)), key=lambda i: (dist_to_puncture(leaves[i][1], leaves[i][2]) / leaves[i][2], i))
-        new = []
-        refined = set()
-        budget = target - len(leaves)
-        for i in order:
-            lvl, lo, side = leaves[i]
-            near = dist_to_puncture(lo, side) < max(refine_r, side)
-            if (near or lvl < 2) and budget >= 7:
-                refined.add(i)
-                budget -= 7
-        if not refined:
-            break
-        for i, (lvl, lo, side) in enumerate(leaves):
-            if i in refined:
-                h = side / 2.0
-                for cz in range(2):
-                    for cy in range(2):
-                        for cx in range(2):
-                            new.append((lvl + 1, (lo[0] + cx * h, lo[1] + cy * h, lo[2] + cz * h), h))
-            else:
-                new.append((lvl, lo, side))
-        leaves = new
-        level += 1
-    leaves.sort(key=lambda t: (t[1][2], t[1][1], t[1][0], -t[2]))
+    def rec(lvl, lo, side):
+        if lvl < lmin or (lvl < lmax and dist(lo, side) < c * side):
+            h = side / 2.0
+            for cz in range(2):
+                for cy in range(2):
+                    for cx in range(2):
+                        rec(lvl + 1, (lo[0] + cx * h, lo[1] + cy * h, lo[2] + cz * h), h)
+        else:
+            leaves.append((lvl, lo, side))
+
+    rec(0, (-domain,) * 3, 2.0 * domain)
     bl = BlockList()
     for bid, (lvl, lo, side) in enumerate(leaves):
         g = splitmix64(0xB10C ^ (bid << 8))
