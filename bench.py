@@ -112,6 +112,54 @@ def normwise_err(a, b):
     return (d / r)
 
 
+def constraint_drift(bl, steps, dx, hev):
+    """Evolve the Brill-Lindquist tiling for `steps` RK4 steps (dt = 0.25 dx,
+    captured RK4-step graph: 4 ghost refreshes + 4 fused stages) in FP64 and in
+    mixed precision; L2 norms of the Hamiltonian and momentum constraints over
+    the interior points every 10 steps (always evaluated in FP64)."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    dt = 0.25 * dx
+    npts = bl.interior_points()
+    res = {"steps": steps, "dt": dt, "cfl": 0.25}
+    for prec in ("fp64", "mixed"):
+        u = data.fill_blocks(bl, backend="torch", device="cuda")
+        sa, sb, acc = torch.zeros_like(u), torch.zeros_like(u), torch.zeros_like(u)
+        c = torch.zeros((4, bl.total), dtype=torch.float64, device="cuda")
+        plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+        cplan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64"))
+        graph = plan.rk4_graph(dt, u, sa, sb, acc)
+        hist = []
+
+        def norms():
+            cplan.ghost_fill(u)
+            cplan.constraints(u, c)
+            l2 = (c.pow(2).sum(dim=1) / npts).sqrt()
+            return {"ham": l2[0].item(), "mom": (l2[1:] ** 2).sum().sqrt().item()}
+        hist.append({"step": 0, **norms()})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(1, steps + 1):
+            graph.launch()
+            if s % 10 == 0:
+                torch.cuda.synchronize()
+                hist.append({"step": s, **norms()})
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        finite = bool(torch.isfinite(u).all().item())
+        res[prec] = {"history": hist, "finite": finite, "wall_s": wall}
+        log(f"drift {prec}: H {hist[0]['ham']:.3e} -> {hist[-1]['ham']:.3e}, M {hist[0]['mom']:.3e} -> "
+            f"{hist[-1]['mom']:.3e}, finite={finite}, {wall:.1f} s")
+        graph.close()
+        del plan, cplan, u, sa, sb, acc, c
+        torch.cuda.empty_cache()
+    hf, hm = res["fp64"]["history"][-1], res["mixed"]["history"][-1]
+    res["ham_l2_rel_diff_mixed_vs_fp64"] = abs(hm["ham"] - hf["ham"]) / max(hf["ham"], 1e-300)
+    res["mom_l2_rel_diff_mixed_vs_fp64"] = abs(hm["mom"] - hf["mom"]) / max(hf["mom"], 1e-300)
+    return res
+
+
 def cpu_baseline(bl, u, target_s=12.0):
     """The CPU oracle (this project's C++ restatement of the reference
     bssnrhs, same structure: full-array sweeps, pointwise loop, BCs, KO) on a
@@ -161,6 +209,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-drift", action="store_true",
+                    help="skip the 100-step constraint-drift comparison of FP64 and mixed (config 3)")
+    ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--no-octree", action="store_true",
                     help="skip the batched uneven-octree measurement (config 2)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -225,6 +276,14 @@ def main():
         "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                         "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
     }
+    drift = None
+    if not args.no_drift:
+        del out, ref_out
+        torch.cuda.empty_cache()
+        drift = constraint_drift(bl, args.drift_steps, 16.0 / (8 * n), hev)
+        out = torch.zeros_like(u)
+        ref_out = out
+
     octree = None
     if not args.no_octree:
         del u, out, ref_out
@@ -249,6 +308,27 @@ def main():
         torch.cuda.empty_cache()
         u = data.fill_blocks(bl, backend="torch", device="cuda")
 
+    # a single large block (192^3 interior, 198^3 padded): the large-block,
+    # single-launch shape (13824 tiles of one block)
+    large = None
+    if not args.no_octree:
+        lbl = data.BlockList()
+        nL = 198
+        lbl.add((nL, nL, nL), (-4.0,) * 3, (4.0,) * 3, 63)
+        lu = data.fill_blocks(lbl, backend="torch", device="cuda", kind="perturbed", seed=7, amp=1e-2,
+                              lo=-4.0, L=8.0)
+        lout = torch.zeros_like(lu)
+        lpts = lbl.interior_points()
+        large = {"workload": "one block of 192^3 interior (198^3 padded), perturbed flat data, all faces flagged",
+                 "interior_points": lpts}
+        for prec in ("fp64", "mixed"):
+            lp = sgr.Plan(lbl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+            w, _, _ = timed_steps(lp, lout, lu, 10, 2, hev)
+            large[prec] = lpts * 10 / w / 1e6
+            del lp
+        log(f"large block: fp64 {large['fp64']:.1f} Mpts/s, mixed {large['mixed']:.1f} Mpts/s")
+        del lu, lout
+
     if args.copy_calibration:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -271,6 +351,7 @@ def main():
                                "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
+        "constraint_drift": drift, "large_block": large,
     }
     print(json.dumps(line), flush=True)
 

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
     __shared__ double DX[TX * HY * HZ];  // D_x f: x in tile, y and z with halo
     __shared__ double DY[TX * TY * HZ];  // D_y f: x, y in tile, z with halo
 
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const BlockDesc &B = blocks[tile_block[tile]];
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const int lt = (int)(tile - B.tile_base);

@@ -75,7 +75,8 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
 
-    const uint32_t tile = tile0 + blockIdx.x;  // chunk-relative launch
+    const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
+    const uint32_t tile = tile0 + ct;
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
@@ -95,8 +96,8 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     bool bpos[2][3];
     // slot-major workspace with a padded plane stride SS (points); this tile's
     // 512 points are contiguous within every plane
-    W *__restrict__ wsb = ws + (uint64_t)blockIdx.x * TILE_PTS;
-    double *__restrict__ kb = wsK + (uint64_t)blockIdx.x * TILE_PTS;
+    W *__restrict__ wsb = ws + (uint64_t)ct * TILE_PTS;
+    double *__restrict__ kb = wsK + (uint64_t)ct * TILE_PTS;
 #pragma unroll
     for (int p = 0; p < 2; p++) {
         pk[p] = tc.k0 + lz + 4 * p;
@@ -301,15 +302,16 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
                                                      const uint32_t *__restrict__ tile_block, const W *__restrict__ ws,
                                                      uint64_t SS, uint32_t tile0)
 {
-    const uint32_t tile = tile0 + (blockIdx.x >> 1);
-    const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = tile0 + (wg >> 1);
+    const int l = (int)((wg & 1) * 256 + threadIdx.x);
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
     const int i = tc.i0 + (l & 7), j = tc.j0 + ((l >> 3) & 7), k = tc.k0 + (l >> 6);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
-    const W *__restrict__ wsb = ws + (uint64_t)(blockIdx.x >> 1) * TILE_PTS + l;
+    const W *__restrict__ wsb = ws + (uint64_t)(wg >> 1) * TILE_PTS + l;
     typedef double R;
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])

@@ -10,8 +10,9 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
                                                    const W *__restrict__ ws, const double *__restrict__ wsK,
                                                    uint64_t SS, Params P, RkArgs rk, uint32_t tile0)
 {
-    const uint32_t tile = tile0 + (blockIdx.x >> 1);  // chunk-relative launch
-    const int l = (int)((blockIdx.x & 1) * 256 + threadIdx.x);
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);  // XCD-contiguous tiles
+    const uint32_t tile = tile0 + (wg >> 1);
+    const int l = (int)((wg & 1) * 256 + threadIdx.x);
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
     const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
@@ -19,8 +20,8 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
     // slot-major workspace, plane stride SS points
-    const W *__restrict__ wsb = ws + (uint64_t)(blockIdx.x >> 1) * TILE_PTS + l;
-    const double *__restrict__ kb = wsK + (uint64_t)(blockIdx.x >> 1) * TILE_PTS + l;
+    const W *__restrict__ wsb = ws + (uint64_t)(wg >> 1) * TILE_PTS + l;
+    const double *__restrict__ kb = wsK + (uint64_t)(wg >> 1) * TILE_PTS + l;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
 
     // coordinates and eta exactly as bssn/src/rhs.cpp:206-219

@@ -179,3 +179,39 @@ def test_octree_uneven_blocks_match_oracle():
         e = max(np.abs(g[f] - r[f]).max() / max(np.abs(r[f]).max(), 1e-300) for f in range(24))
         print(f"octree block {b} size {s[0]} bflag {bl.bflag[b]} err {e:.2e}", flush=True)
         assert e <= 1e-12
+
+
+def test_large_single_block_matches_sub_blocks():
+    """One 192^3-interior block (198^3 padded, 13824 tiles in one launch)
+    against 38^3 sub-blocks cut from the same data at three places (the big
+    block's face closures included through bflag=63 corners)."""
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    torch = __import__("torch")
+    n = 198
+    lo, h = -4.0, 8.0 / 197
+    big = data.BlockList()
+    big.add((n, n, n), (lo,) * 3, (lo + (n - 1) * h,) * 3, 63)
+    u = data.fill_blocks(big, backend="torch", device="cuda", kind="perturbed", seed=7, amp=1e-2, lo=lo, L=8.0)
+    out = torch.zeros_like(u)
+    plan = sgr.Plan(big, sgr.params(adv="upwind"))
+    plan.rhs(out, u)
+    torch.cuda.synchronize()
+    U = u.view(24, n, n, n)
+    O = out.view(24, n, n, n)
+    for (i0, j0, k0) in ((0, 0, 0), (80, 64, 96), (160, 160, 160)):
+        m = 38
+        sub = data.BlockList()
+        flags = (1 if i0 == 0 else 0) | (4 if j0 == 0 else 0) | (16 if k0 == 0 else 0) | \
+                (2 if i0 + m == n else 0) | (8 if j0 + m == n else 0) | (32 if k0 + m == n else 0)
+        pmin = (lo + i0 * h, lo + j0 * h, lo + k0 * h)
+        sub.add((m, m, m), pmin, tuple(p + (m - 1) * h for p in pmin), flags)
+        su = U[:, k0:k0 + m, j0:j0 + m, i0:i0 + m].contiguous().view(24, -1)
+        so = torch.zeros_like(su)
+        sgr.Plan(sub, sgr.params(adv="upwind")).rhs(so, su)
+        torch.cuda.synchronize()
+        a = so.view(24, m, m, m)[:, 3:-3, 3:-3, 3:-3]
+        b = O[:, k0 + 3:k0 + m - 3, j0 + 3:j0 + m - 3, i0 + 3:i0 + m - 3]
+        err = ((a - b).abs().amax(dim=(1, 2, 3)) / b.abs().amax(dim=(1, 2, 3)).clamp_min(1e-300)).max().item()
+        print(f"sub-block at {(i0, j0, k0)} flags {flags} err {err:.2e}", flush=True)
+        assert err <= 1e-12
