@@ -110,14 +110,23 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
  * generated counterpart (bssn/CodeGen/physcon.cpp) does not compile. */
 int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream);
 
-/* Same-level ghost refresh (the unzip step between RK stages): every ghost
- * point of every plan block whose neighbour region is another plan block of
- * the same size and spacing is copied from that block's interior; ghost
- * regions beyond flagged (domain-boundary) faces are left as they are (no
- * stencil reads them).  SGR_E_UNSUPPORTED if the plan has a level boundary
- * (2:1 prolongation is not provided).  No reference counterpart (Dendro's
- * unzip lives outside the reference repository). */
+/* Ghost refresh (the unzip step between RK stages), in place.  Every block
+ * owns the box of its interior nodes widened by half a cell; a ghost node
+ * inside another block's owned box takes that block's value: the coinciding
+ * node where the grids align, otherwise the tensor-product 6-point Lagrange
+ * interpolant over that block's interior nodes (neighbours may differ in
+ * spacing, size and level).  Ghost nodes beyond flagged (domain-boundary)
+ * faces are left as they are (no stencil reads them).  SGR_E_UNSUPPORTED if
+ * some ghost node inside the domain has no owner, blocks overlap, or the
+ * block coordinates share no common lattice.  No reference counterpart
+ * (Dendro's unzip lives outside the reference repository). */
 int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t stream);
+
+/* Host-only check of the ghost refresh table sgr_ghost_fill would use for
+ * these blocks (no GPU needed): stats[0] patches, stats[1] ghost nodes
+ * refreshed, stats[2] of them interpolated (not plain copies).  Returns the
+ * error sgr_ghost_fill would return.  No reference counterpart. */
+int sgr_ghost_table_stats(const sgr_block *blocks, uint32_t nblocks, uint64_t stats[3]);
 
 /* One classic-RK4 stage with the update fused into the RHS output store
  * (interior points only; a ghost refresh must precede the next stage):

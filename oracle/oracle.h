@@ -45,6 +45,13 @@ int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t nblocks,
 int oracle_constraints(double **out, const double **in, uint64_t offset, const double *pmin, const double *pmax,
                        const uint32_t *sz, uint32_t bflag);
 
+/* Ghost refresh of a block set in place (oracle/ghost_oracle.cpp): every
+ * ghost node owned by another block's interior box takes that block's value,
+ * coinciding node or 6-point Lagrange interpolant per axis.  -1 when the
+ * block coordinates share no common lattice. */
+int oracle_ghost_fill(double **u, uint32_t nblocks, const uint64_t *offsets, const uint32_t *sz,
+                      const double *pmin, const double *pmax, const uint32_t *bflag);
+
 /* One stencil sweep: which = 0..2 first x/y/z, 3..5 second xx/yy/zz,
  * 6..8 advective x/y/z (needs beta_d), 9..11 Kreiss-Oliger x/y/z.
  * upwind selects the advective stencil as in oracle_params. */

@@ -56,6 +56,8 @@ def lib():
                                             ctypes.POINTER(OracleParams), ctypes.c_int]
         L.oracle_constraints.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]
+        L.oracle_ghost_fill.argtypes = [ctypes.POINTER(P_D), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint32), P_D, P_D, ctypes.POINTER(ctypes.c_uint32)]
         L.oracle_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), P_D,
                                    ctypes.c_uint32, ctypes.c_int]
         _lib = L
@@ -207,9 +209,26 @@ def ghost_fill_uniform(u, bl, nb):
     return out
 
 
+def ghost_fill(u, bl):
+    """Ghost refresh of any block set (oracle_ghost_fill); returns a copy."""
+    nb = len(bl)
+    out = np.ascontiguousarray(u, dtype=np.float64).copy()
+    offs = (ctypes.c_uint64 * nb)(*bl.offsets)
+    pmin = (ctypes.c_double * (3 * nb))(*[v for p in bl.pmin for v in p])
+    pmax = (ctypes.c_double * (3 * nb))(*[v for p in bl.pmax for v in p])
+    sz = (ctypes.c_uint32 * (3 * nb))(*[v for s in bl.sz for v in s])
+    bf = (ctypes.c_uint32 * nb)(*bl.bflag)
+    rc = lib().oracle_ghost_fill(_ptrs(out), nb, offs, sz, pmin, pmax, bf)
+    if rc != 0:
+        raise RuntimeError(f"oracle_ghost_fill failed: {rc}")
+    return out
+
+
 def rk4_step(u, bl, nb, dt, params, nthreads=1):
     """Classic RK4 step of the oracle RHS with ghost refresh before every
-    stage; only interior points change (textbook tableau)."""
+    stage; only interior points change (textbook tableau).  nb: the side of an
+    nb^3 uniform tiling (lattice-index ghost copy), or None for any block set
+    (oracle_ghost_fill)."""
     def interior_mask():
         mk = np.zeros(u.shape[1], dtype=bool)
         for b in range(len(bl)):
@@ -219,20 +238,23 @@ def rk4_step(u, bl, nb, dt, params, nthreads=1):
             mk[bl.offsets[b]:bl.offsets[b] + n ** 3] = v.reshape(-1)
         return mk
     mk = interior_mask()
-    un = ghost_fill_uniform(u, bl, nb)
+
+    def fill(v):
+        return ghost_fill_uniform(v, bl, nb) if nb is not None else ghost_fill(v, bl)
+    un = fill(u)
     k1 = rhs_blocks(un, bl, params, nthreads)
     acc = un.copy(); s = un.copy()
     acc[:, mk] = un[:, mk] + (dt / 6) * k1[:, mk]
     s[:, mk] = un[:, mk] + (dt / 2) * k1[:, mk]
-    s = ghost_fill_uniform(s, bl, nb)
+    s = fill(s)
     k2 = rhs_blocks(s, bl, params, nthreads)
     acc[:, mk] = acc[:, mk] + (dt / 3) * k2[:, mk]
     s2 = s.copy(); s2[:, mk] = un[:, mk] + (dt / 2) * k2[:, mk]
-    s2 = ghost_fill_uniform(s2, bl, nb)
+    s2 = fill(s2)
     k3 = rhs_blocks(s2, bl, params, nthreads)
     acc[:, mk] = acc[:, mk] + (dt / 3) * k3[:, mk]
     s3 = s2.copy(); s3[:, mk] = un[:, mk] + dt * k3[:, mk]
-    s3 = ghost_fill_uniform(s3, bl, nb)
+    s3 = fill(s3)
     k4 = rhs_blocks(s3, bl, params, nthreads)
     new = un.copy()
     new[:, mk] = acc[:, mk] + (dt / 6) * k4[:, mk]

@@ -92,6 +92,8 @@ _SIGS = {
                                       ctypes.POINTER(ctypes.c_void_p)]),
     "sgr_constraints": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_ghost_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_ghost_table_stats": (ctypes.c_int, [ctypes.POINTER(SgrBlock), ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_uint64)]),
     "sgr_rk4_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(P_D),
                                      ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_rk4_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(P_D), ctypes.POINTER(P_D),
@@ -158,6 +160,15 @@ def block_array(bl):
     return arr
 
 
+def ghost_table_stats(bl):
+    """Host-only check of the ghost refresh table for a BlockList:
+    {"patches", "nodes", "interpolated"}; raises SgrError (UNSUPPORTED) when
+    the refresh would be refused."""
+    st = (ctypes.c_uint64 * 3)()
+    check(lib().sgr_ghost_table_stats(block_array(bl), len(bl), st))
+    return {"patches": st[0], "nodes": st[1], "interpolated": st[2]}
+
+
 def _field_ptrs(t):
     """24 device pointers of the rows of a (24, total) tensor."""
     if t.dim() != 2 or t.shape[0] != NUM_VARS:
@@ -218,7 +229,8 @@ class Plan:
         check(lib().sgr_constraints(self._h, _field_ptrs(u), po, ctypes.c_void_p(self._stream(u, stream))))
 
     def ghost_fill(self, u, stream=None):
-        """Same-level ghost refresh of u (in place)."""
+        """Ghost refresh of u in place (copies where grids align, 6-point
+        Lagrange interpolation from the owning block elsewhere)."""
         check(lib().sgr_ghost_fill(self._h, _field_ptrs(u), ctypes.c_void_p(self._stream(u, stream))))
 
     def rk4_stage(self, stage, dt, u_n, u_stage, acc, u_next, stream=None):

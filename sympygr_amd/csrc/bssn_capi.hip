@@ -2,6 +2,7 @@
 // workspace allocated once), the launch entry points and error reporting.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -117,6 +118,14 @@ __global__ void k_check_finite(FieldPtrs in, const BlockDesc *blocks, uint32_t n
 
 }  // namespace
 
+struct GhostTable {
+    std::vector<GhostPatch> patches;  // sorted by class
+    uint32_t start[GHOST_CLASSES + 1] = {};
+    uint64_t nodes = 0, interp_nodes = 0;
+    bool complete = true;
+    std::string msg;
+};
+
 enum class Variant { Fused, Workspace };
 
 struct sgr_plan {
@@ -135,78 +144,160 @@ struct sgr_plan {
     uint64_t ws_bytes = 0;
     uint64_t interior = 0;
     uint32_t nslots = 0;
-    // same-level ghost refresh table
-    std::vector<GhostCopy> hghost;
-    GhostCopy *d_ghost = nullptr;
-    bool ghost_complete = true;
-    std::string ghost_msg;
+    // ghost refresh table
+    GhostTable ghost;
+    GhostPatch *d_ghost = nullptr;
 };
 
-// Same-level neighbour copies for every ghost box of every block.  Blocks are
-// matched on integer lattice positions of their interior origins (in units of
-// their own spacing) and must share sizes and spacings; a ghost box with no
-// neighbour is skipped when it lies beyond a flagged (domain-boundary) face,
-// otherwise the table is marked incomplete (level boundary: needs
-// prolongation, not provided).
-static void build_ghost_table(sgr_plan *p, const sgr_block *blocks, uint32_t nb)
+// Ghost patch table of a block set (host only).
+//
+// Every block owns the half-open box of its interior nodes widened by half a
+// cell.  Positions are taken to exact integers on the coarsest lattice of unit
+// hmin / (2K), K = 1..64, on which every block origin and every half spacing
+// is an integer, so ownership has no rounding ties.  For each block, every
+// other block whose owned box meets its padded node box contributes one patch:
+// the box of the target's nodes inside the source's owned box, clipped to
+// the target's ghost region that is not beyond a flagged (domain-boundary)
+// face.  The table is complete when these patches cover exactly the target's
+// ghost nodes inside the domain; otherwise the reason is reported (nodes with
+// no owner, overlapping blocks, no common lattice).
+
+static int64_t ceil_div(int64_t a, int64_t b)  // b > 0
 {
-    if (nb == 0) return;
-    double h0[3], o0[3];
-    for (int d = 0; d < 3; d++) {
-        h0[d] = (blocks[0].pmax[d] - blocks[0].pmin[d]) / (blocks[0].sz[d] - 1);
-        o0[d] = blocks[0].pmin[d];
-    }
-    std::map<std::tuple<long long, long long, long long, long long>, uint32_t> at;
-    auto key = [&](const sgr_block &b, int dx, int dy, int dz) {
-        long long k[3];
-        const int dir[3] = {dx, dy, dz};
-        for (int d = 0; d < 3; d++) {
-            const double h = (b.pmax[d] - b.pmin[d]) / (b.sz[d] - 1);
-            const double o = b.pmin[d] + dir[d] * (double)(b.sz[d] - 6) * h;
-            k[d] = llround((o - o0[d]) / h0[d] * 16.0);  // 1/16-cell lattice of block 0
-        }
-        const double h = (b.pmax[0] - b.pmin[0]) / (b.sz[0] - 1);
-        return std::make_tuple(k[0], k[1], k[2], llround(h / h0[0] * 1024.0));
+    return a >= 0 ? (a + b - 1) / b : -((-a) / b);
+}
+
+static GhostTable build_ghost_table(const sgr_block *blocks, uint32_t nb)
+{
+    GhostTable T;
+    auto incomplete = [&](const std::string &m) {
+        if (T.complete) T.complete = false, T.msg = m;
     };
-    for (uint32_t b = 0; b < nb; b++) at[key(blocks[b], 0, 0, 0)] = b;
-    for (uint32_t b = 0; b < nb; b++) {
-        const sgr_block &B = blocks[b];
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dx = -1; dx <= 1; dx++) {
-                    if (!dx && !dy && !dz) continue;
-                    const int dir[3] = {dx, dy, dz};
-                    auto it = at.find(key(B, dx, dy, dz));
-                    bool ok = it != at.end();
-                    if (ok) {
-                        const sgr_block &N = blocks[it->second];
-                        ok = N.sz[0] == B.sz[0] && N.sz[1] == B.sz[1] && N.sz[2] == B.sz[2];
-                    }
-                    if (!ok) {
-                        bool outside = false;
-                        for (int d = 0; d < 3; d++)
-                            if (dir[d] && (B.bflag & (1u << (2 * d + (dir[d] > 0 ? 1 : 0))))) outside = true;
-                        if (!outside && p->ghost_complete) {
-                            p->ghost_complete = false;
-                            p->ghost_msg = "block " + std::to_string(b) +
-                                           " has a ghost region with no same-level neighbour (level boundary)";
-                        }
-                        continue;
-                    }
-                    GhostCopy g{};
-                    g.dst_off = B.offset;
-                    g.src_off = blocks[it->second].offset;
-                    g.n0 = (int32_t)B.sz[0];
-                    g.n1 = (int32_t)B.sz[1];
-                    for (int d = 0; d < 3; d++) {
-                        const int n = (int)B.sz[d];
-                        g.d[d] = dir[d] < 0 ? 0 : (dir[d] == 0 ? 3 : n - 3);
-                        g.s[d] = dir[d] < 0 ? n - 6 : 3;
-                        g.e[d] = dir[d] == 0 ? n - 6 : 3;
-                    }
-                    p->hghost.push_back(g);
+    if (nb == 0) return T;
+    struct Lat {
+        int64_t P[3], H[3], m[3], lo[3], hi[3];
+    };
+    std::vector<Lat> L(nb);
+    double O[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hmin = HUGE_VAL;
+    for (uint32_t b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++) {
+            O[d] = std::fmin(O[d], blocks[b].pmin[d]);
+            hmin = std::fmin(hmin, (blocks[b].pmax[d] - blocks[b].pmin[d]) / (blocks[b].sz[d] - 1));
+        }
+    bool found = false;
+    for (int K = 1; K <= 64 && !found; K++) {
+        const double unit = hmin / (2.0 * K);
+        found = true;
+        for (uint32_t b = 0; b < nb && found; b++)
+            for (int d = 0; d < 3 && found; d++) {
+                const double h = (blocks[b].pmax[d] - blocks[b].pmin[d]) / (blocks[b].sz[d] - 1);
+                const double a = (blocks[b].pmin[d] - O[d]) / unit, c = h / (2.0 * unit);
+                if (std::fabs(a - std::nearbyint(a)) > 1e-6 || std::fabs(c - std::nearbyint(c)) > 1e-6 ||
+                    std::fabs(a) > 1e9 || c > 1e9) {
+                    found = false;
+                    break;
                 }
+                Lat &l = L[b];
+                l.P[d] = (int64_t)std::nearbyint(a);
+                l.H[d] = 2 * (int64_t)std::nearbyint(c);
+                l.m[d] = blocks[b].sz[d];
+                l.lo[d] = l.P[d] + 5 * (l.H[d] / 2);
+                l.hi[d] = l.P[d] + (l.m[d] - 4) * l.H[d] + l.H[d] / 2;
+            }
     }
+    if (!found) {
+        incomplete("block coordinates share no common lattice (origins and half spacings must be integer "
+                   "multiples of one unit)");
+        return T;
+    }
+    for (uint32_t b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++)
+            if (L[b].P[d] + L[b].m[d] * L[b].H[d] >= (int64_t(1) << 30)) {
+                incomplete("block set spans more than 2^30 lattice units");
+                return T;
+            }
+    // candidate sources of a target: owned boxes sorted by their low x
+    std::vector<uint32_t> order(nb);
+    for (uint32_t b = 0; b < nb; b++) order[b] = b;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return L[a].lo[0] < L[c].lo[0]; });
+    int64_t wmax = 0;
+    for (uint32_t b = 0; b < nb; b++) wmax = std::max(wmax, L[b].hi[0] - L[b].lo[0]);
+    std::vector<GhostPatch> cls[GHOST_CLASSES];
+    for (uint32_t b = 0; b < nb && T.complete; b++) {
+        const Lat &t = L[b];
+        int64_t al[3], ah[3];
+        uint64_t required = 1, interior = 1, covered = 0;
+        for (int d = 0; d < 3; d++) {
+            al[d] = (blocks[b].bflag >> (2 * d)) & 1 ? 3 : 0;
+            ah[d] = (blocks[b].bflag >> (2 * d + 1)) & 1 ? t.m[d] - 3 : t.m[d];
+            required *= (uint64_t)(ah[d] - al[d]);
+            interior *= (uint64_t)(t.m[d] - 6);
+        }
+        required -= interior;
+        const int64_t x0 = t.P[0], x1 = t.P[0] + (t.m[0] - 1) * t.H[0];
+        auto it = std::lower_bound(order.begin(), order.end(), x0 - wmax,
+                                   [&](uint32_t s, int64_t v) { return L[s].lo[0] < v; });
+        for (; it != order.end() && L[*it].lo[0] <= x1; ++it) {
+            const uint32_t sidx = *it;
+            if (sidx == b) continue;
+            const Lat &s = L[sidx];
+            int64_t lo[3], hi[3];
+            bool empty = false;
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::max(al[d], ceil_div(s.lo[d] - t.P[d], t.H[d]));
+                hi[d] = std::min(ah[d], ceil_div(s.hi[d] - t.P[d], t.H[d]));
+                empty |= lo[d] >= hi[d];
+            }
+            if (empty) continue;
+            bool meets_interior = true;
+            for (int d = 0; d < 3; d++) meets_interior &= lo[d] < t.m[d] - 3 && hi[d] > 3;
+            if (meets_interior) {
+                incomplete("blocks " + std::to_string(b) + " and " + std::to_string(sidx) + " overlap");
+                break;
+            }
+            GhostPatch g{};
+            g.dst_off = blocks[b].offset;
+            g.src_off = blocks[sidx].offset;
+            g.dn0 = (int32_t)t.m[0];
+            g.dn1 = (int32_t)t.m[1];
+            g.sn0 = (int32_t)s.m[0];
+            g.sn1 = (int32_t)s.m[1];
+            int c = 0;
+            uint64_t vol = 1;
+            for (int d = 0; d < 3; d++) {
+                g.lo[d] = (int32_t)lo[d];
+                g.e[d] = (int32_t)(hi[d] - lo[d]);
+                g.rel[d] = (int32_t)(t.P[d] - s.P[d]);
+                g.ht[d] = (int32_t)t.H[d];
+                g.hs[d] = (int32_t)s.H[d];
+                g.sm[d] = (int32_t)s.m[d];
+                g.inv_hs[d] = 1.0 / (double)s.H[d];
+                if (t.H[d] % s.H[d] == 0 && (t.P[d] - s.P[d]) % s.H[d] == 0) c |= 1 << d;
+                vol *= (uint64_t)g.e[d];
+                // every tap an interior source node (owned box => q in [2, m-4])
+                const int64_t lat0 = g.rel[d] + lo[d] * t.H[d], lat1 = g.rel[d] + (hi[d] - 1) * t.H[d];
+                if (lat0 / s.H[d] < 2 || lat1 / s.H[d] > s.m[d] - 4 || s.m[d] < 12) {
+                    incomplete("ghost patch of block " + std::to_string(b) + " leaves the interior of block " +
+                               std::to_string(sidx));
+                }
+            }
+            covered += vol;
+            T.nodes += vol;
+            if (c != GHOST_CLASSES - 1) T.interp_nodes += vol;
+            cls[c].push_back(g);
+        }
+        if (T.complete && covered != required)
+            incomplete("block " + std::to_string(b) + ": " + std::to_string(covered) + " of " +
+                       std::to_string(required) + " ghost nodes inside the domain have an owning block" +
+                       (covered > required ? " (overlapping blocks)" : " (the rest lie outside every block)"));
+    }
+    if (!T.complete) return T;
+    for (int c = 0; c < GHOST_CLASSES; c++) {
+        T.start[c] = (uint32_t)T.patches.size();
+        T.patches.insert(T.patches.end(), cls[c].begin(), cls[c].end());
+    }
+    T.start[GHOST_CLASSES] = (uint32_t)T.patches.size();
+    return T;
 }
 
 static void plan_free(sgr_plan *p)
@@ -348,12 +439,11 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         }
     }
     if ((e = hipMalloc(&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
-    build_ghost_table(p, blocks, nblocks);
-    if (!p->hghost.empty()) {
-        if ((e = hipMalloc(&p->d_ghost, p->hghost.size() * sizeof(GhostCopy))) != hipSuccess)
-            return bail(e, "hipMalloc(ghost table)");
-        if ((e = hipMemcpy(p->d_ghost, p->hghost.data(), p->hghost.size() * sizeof(GhostCopy),
-                           hipMemcpyHostToDevice)) != hipSuccess)
+    p->ghost = build_ghost_table(blocks, nblocks);
+    if (!p->ghost.patches.empty()) {
+        const size_t gb = p->ghost.patches.size() * sizeof(GhostPatch);
+        if ((e = hipMalloc(&p->d_ghost, gb)) != hipSuccess) return bail(e, "hipMalloc(ghost table)");
+        if ((e = hipMemcpy(p->d_ghost, p->ghost.patches.data(), gb, hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(ghost table)");
     }
     hipSetDevice(cur);
@@ -487,7 +577,7 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
 int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t stream)
 {
     if (!plan || !u) return fail(SGR_E_INVALID, "null argument");
-    if (!plan->ghost_complete) return fail(SGR_E_UNSUPPORTED, plan->ghost_msg);
+    if (!plan->ghost.complete) return fail(SGR_E_UNSUPPORTED, plan->ghost.msg);
     OutPtrs o;
     for (int f = 0; f < NF; f++) {
         if (!u[f]) return fail(SGR_E_INVALID, "null field pointer");
@@ -496,9 +586,24 @@ int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t st
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = launch_ghost_fill(o, plan->d_ghost, (uint32_t)plan->hghost.size(), stream);
+    hipError_t e = launch_ghost_fill(o, plan->d_ghost, plan->ghost.start, stream);
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "ghost fill launch");
+    return SGR_OK;
+}
+
+int sgr_ghost_table_stats(const sgr_block *blocks, uint32_t nblocks, uint64_t stats[3])
+{
+    if (!stats || (nblocks && !blocks)) return fail(SGR_E_INVALID, "null argument");
+    for (uint32_t b = 0; b < nblocks; b++) {
+        const int rc = validate_block(blocks[b], b);
+        if (rc) return rc;
+    }
+    const GhostTable T = build_ghost_table(blocks, nblocks);
+    stats[0] = T.patches.size();
+    stats[1] = T.nodes;
+    stats[2] = T.interp_nodes;
+    if (!T.complete) return fail(SGR_E_UNSUPPORTED, T.msg);
     return SGR_OK;
 }
 

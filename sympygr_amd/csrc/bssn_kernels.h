@@ -35,20 +35,33 @@ struct LaunchArgs {
     RkArgs rk;                   // rk.stage < 0: plain RHS
 };
 
-// Same-level ghost refresh: copy one box of ghost points of a block from the
-// interior of its neighbour (all 24 fields).
-struct GhostCopy {
+// Ghost refresh: one box of ghost nodes of a target block whose positions lie
+// in the owned box of one source block (its interior nodes widened by half a
+// cell).  Positions are integers on a lattice common to all blocks: target
+// node i sits at rel + i*ht relative to source node 0, source nodes are hs
+// apart.  Per axis a ghost node takes the coinciding source node or the
+// 6-point Lagrange interpolant over source interior nodes.
+struct GhostPatch {
     uint64_t dst_off, src_off;   // block offsets in the field arrays
-    int32_t n0, n1;              // padded x, y sizes (shared by both blocks)
-    int32_t d[3];                // first destination point (padded coords)
-    int32_t s[3];                // first source point
-    int32_t e[3];                // extents
+    int32_t dn0, dn1;            // target padded x, y sizes
+    int32_t sn0, sn1;            // source padded x, y sizes
+    int32_t lo[3], e[3];         // target node box (padded coords) and extents
+    int32_t rel[3];              // target node 0 - source node 0 (lattice units)
+    int32_t ht[3], hs[3];        // spacings (lattice units)
+    int32_t sm[3];               // source padded sizes
+    double inv_hs[3];            // 1.0 / hs
 };
+
+// Patch classes: bit d set when every target node of the box coincides with
+// a source node along axis d (one tap instead of six).  Patches are stored
+// sorted by class; class c occupies [start[c], start[c+1]).
+constexpr int GHOST_CLASSES = 8;
 
 // Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints; uses the workspace.
 hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s);
 
-hipError_t launch_ghost_fill(OutPtrs u, const GhostCopy *copies, uint32_t ncopies, hipStream_t s);
+hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const uint32_t start[GHOST_CLASSES + 1],
+                             hipStream_t s);
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);    // derivative workspace + pointwise
 hipError_t launch_fused(const LaunchArgs &a, hipStream_t s);  // fused single pass

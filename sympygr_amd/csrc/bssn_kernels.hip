@@ -346,24 +346,100 @@ hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
     return a.upwind ? launch_constraints_t<double, true>(a, cout, s) : launch_constraints_t<double, false>(a, cout, s);
 }
 
-// One workgroup per copy box; threads walk the box, all 24 fields.
-__global__ __launch_bounds__(256) void k_ghost_fill(OutPtrs u, const GhostCopy *__restrict__ copies)
+// Taps of one axis for target node `idx`: returns the first source node and
+// fills NT weights.  NT == 1: the node coincides with source node q.  NT == 6:
+// 6-point Lagrange over p0..p0+5, p0 = clamp(q - 2, 3, sm - 9), at
+// t = (q - p0) + rem / hs; a node that happens to coincide gets the unit
+// weight vector.  The position lies in the source's owned box, so
+// q is in [2, sm - 4] and every tap is an interior node.
+template <int NT>
+__device__ inline int ghost_axis(int idx, int rel, int ht, int hs, double inv_hs, int sm, double *w)
 {
-    const GhostCopy c = copies[blockIdx.x];
-    const int npts = c.e[0] * c.e[1] * c.e[2];
+    SGR_EXACT
+    const int lat = rel + idx * ht;
+    const int q = lat / hs, rem = lat - q * hs;
+    if (NT == 1) {
+        w[0] = 1.0;
+        return q;
+    }
+    const int p0 = min(max(q - 2, 3), sm - 9);
+    if (rem == 0) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) w[j] = (j == q - p0) ? 1.0 : 0.0;
+        return p0;
+    }
+    constexpr double C[6] = {1.0 / -120.0, 1.0 / 24.0, 1.0 / -12.0, 1.0 / 12.0, 1.0 / -24.0, 1.0 / 120.0};
+    const double t = (double)(q - p0) + (double)rem * inv_hs;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        double N = 1.0;
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+            if (k != j) N = N * (t - (double)k);
+        w[j] = N * C[j];
+    }
+    return p0;
+}
+
+// One workgroup per patch; threads walk the patch's ghost nodes and refresh
+// all 24 fields in place (sources are interior nodes, targets ghost nodes).
+// Sum order sum_c wz_c (sum_b wy_b (sum_a wx_a u)), contraction off.
+template <int NX, int NY, int NZ>
+__global__ __launch_bounds__(256) void k_ghost_fill(OutPtrs u, const GhostPatch *__restrict__ patches)
+{
+    SGR_EXACT
+    const GhostPatch g = patches[blockIdx.x];
+    const int npts = g.e[0] * g.e[1] * g.e[2];
     for (int q = threadIdx.x; q < npts; q += 256) {
-        const int x = q % c.e[0], y = (q / c.e[0]) % c.e[1], z = q / (c.e[0] * c.e[1]);
-        const uint64_t dp = c.dst_off + (c.d[0] + x) + (uint64_t)c.n0 * ((c.d[1] + y) + (uint64_t)c.n1 * (c.d[2] + z));
-        const uint64_t sp = c.src_off + (c.s[0] + x) + (uint64_t)c.n0 * ((c.s[1] + y) + (uint64_t)c.n1 * (c.s[2] + z));
-#pragma unroll 4
-        for (int f = 0; f < NF; f++) u.p[f][dp] = u.p[f][sp];
+        const int x = g.lo[0] + q % g.e[0], y = g.lo[1] + (q / g.e[0]) % g.e[1], z = g.lo[2] + q / (g.e[0] * g.e[1]);
+        double wx[NX], wy[NY], wz[NZ];
+        const int ax = ghost_axis<NX>(x, g.rel[0], g.ht[0], g.hs[0], g.inv_hs[0], g.sm[0], wx);
+        const int ay = ghost_axis<NY>(y, g.rel[1], g.ht[1], g.hs[1], g.inv_hs[1], g.sm[1], wy);
+        const int az = ghost_axis<NZ>(z, g.rel[2], g.ht[2], g.hs[2], g.inv_hs[2], g.sm[2], wz);
+        const uint64_t src = g.src_off + ax + (uint64_t)g.sn0 * (ay + (uint64_t)g.sn1 * az);
+        const uint64_t dst = g.dst_off + x + (uint64_t)g.dn0 * (y + (uint64_t)g.dn1 * z);
+        const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
+#pragma unroll 1
+        for (int f = 0; f < NF; f++) {
+            const double *__restrict__ v = u.p[f] + src;
+            double out = 0.0;
+#pragma unroll
+            for (int c = 0; c < NZ; c++) {
+                double plane = 0.0;
+#pragma unroll
+                for (int b = 0; b < NY; b++) {
+                    double row = 0.0;
+#pragma unroll
+                    for (int a = 0; a < NX; a++) row = row + wx[a] * v[a + b * sy + c * sz];
+                    plane = plane + wy[b] * row;
+                }
+                out = out + wz[c] * plane;
+            }
+            u.p[f][dst] = out;
+        }
     }
 }
 
-hipError_t launch_ghost_fill(OutPtrs u, const GhostCopy *copies, uint32_t ncopies, hipStream_t s)
+template <int CLS>
+static void launch_ghost_class(OutPtrs u, const GhostPatch *patches, const uint32_t *start, hipStream_t s)
 {
-    if (ncopies == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ghost_fill, dim3(ncopies), dim3(256), 0, s, u, copies);
+    const uint32_t n = start[CLS + 1] - start[CLS];
+    if (n == 0) return;
+    constexpr int NX = (CLS & 1) ? 1 : 6, NY = (CLS & 2) ? 1 : 6, NZ = (CLS & 4) ? 1 : 6;
+    hipLaunchKernelGGL((k_ghost_fill<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, patches + start[CLS]);
+}
+
+hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const uint32_t start[GHOST_CLASSES + 1],
+                             hipStream_t s)
+{
+    launch_ghost_class<0>(u, patches, start, s);
+    launch_ghost_class<1>(u, patches, start, s);
+    launch_ghost_class<2>(u, patches, start, s);
+    launch_ghost_class<3>(u, patches, start, s);
+    launch_ghost_class<4>(u, patches, start, s);
+    launch_ghost_class<5>(u, patches, start, s);
+    launch_ghost_class<6>(u, patches, start, s);
+    launch_ghost_class<7>(u, patches, start, s);
     return hipGetLastError();
 }
 

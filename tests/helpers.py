@@ -51,3 +51,58 @@ def case_params(c, **kw):
     p = dict(m["params"])
     p.update(kw)
     return p, c["variant"] == "u"
+
+
+def small_octree(sizes=(16, 24)):
+    """22 leaves on levels 1..3 of [-2, 2]^3 refined around one point: leaves
+    of interior 16 or 24, neighbours of other sizes and levels (4:1 spacing
+    jumps included) - every kind of ghost patch in a set the oracle refreshes
+    in well under a second."""
+    from sympygr_amd import data
+    return data.octree_blocks(domain=2.0, centres=((0.3, 0.2, 0.1),), c=0.01, lmin=1, lmax=3, sizes=sizes)
+
+
+def ghost_mask(bl, inside_only=True):
+    """Boolean mask over the field arrays: ghost nodes (inside the domain, i.e.
+    not beyond a flagged face, when inside_only)."""
+    mk = np.zeros(bl.total, dtype=bool)
+    for b in range(len(bl)):
+        s = bl.sz[b]
+        v = np.ones((s[2], s[1], s[0]), dtype=bool)
+        v[3:-3, 3:-3, 3:-3] = False
+        if inside_only:
+            f = bl.bflag[b]
+            if f & 1:
+                v[:, :, :3] = False
+            if f & 2:
+                v[:, :, -3:] = False
+            if f & 4:
+                v[:, :3, :] = False
+            if f & 8:
+                v[:, -3:, :] = False
+            if f & 16:
+                v[:3, :, :] = False
+            if f & 32:
+                v[-3:, :, :] = False
+        mk[bl.offsets[b]:bl.offsets[b] + v.size] = v.reshape(-1)
+    return mk
+
+
+def coords(bl):
+    """x, y, z of every node of every block, flattened like the field arrays."""
+    from sympygr_amd import data
+    X = np.empty((3, bl.total))
+    for b in range(len(bl)):
+        o, s = bl.offsets[b], bl.sz[b]
+        n = s[0] * s[1] * s[2]
+        x, y, z = data.block_coords(data._Np(), s, bl.pmin[b], bl.pmax[b])
+        X[0, o:o + n], X[1, o:o + n], X[2, o:o + n] = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+    return X
+
+
+def poly5(X, f):
+    """A field-dependent polynomial of degree 5 in each coordinate."""
+    x, y, z = X / 2.0
+    a = 0.1 * (f + 1)
+    return (1.0 + a * x - 0.3 * x ** 2 + 0.07 * x ** 5) * (0.5 - y ** 3 + a * y ** 4 - 0.2 * y ** 5) * \
+        (2.0 + z - a * z ** 2 + 0.4 * z ** 5) + 0.25 * x ** 5 * z ** 4
