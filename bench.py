@@ -14,7 +14,11 @@ RHS evaluation of all 512 blocks with the inputs resident in HBM.
 rate (FP32 pointwise algebra) with its measured error against FP64, the
 roofline of the RHS pass from HIP events recorded around each kernel inside
 the timed loop, and the CPU oracle timed on a bounded sample of the same
-blocks on this host.
+blocks on this host.  Beside it (skipped with --no-octree): the batched
+uneven octree (3032 leaves of interior 16/24/32/48 over [-16,16]^3, one RHS
+in FP64 and mixed with the mixed error, the captured RK4 step with its ghost
+refresh across leaves, the FP64-vs-mixed constraint drift over --drift-steps
+RK4 steps) and one 192^3-interior block.
 """
 from __future__ import annotations
 
@@ -112,52 +116,151 @@ def normwise_err(a, b):
     return (d / r)
 
 
-def constraint_drift(bl, steps, dx, hev):
-    """Evolve the Brill-Lindquist tiling for `steps` RK4 steps (dt = 0.25 dx,
-    captured RK4-step graph: 4 ghost refreshes + 4 fused stages) in FP64 and in
-    mixed precision; L2 norms of the Hamiltonian and momentum constraints over
-    the interior points every 10 steps (always evaluated in FP64)."""
+def block_volume_weights(bl, device):
+    """h^3 at every node of every block (flattened like the field arrays), for
+    volume-weighted norms over blocks of different spacings."""
+    import torch
+    w = torch.empty(bl.total, dtype=torch.float64, device=device)
+    for b in range(len(bl)):
+        s = bl.sz[b]
+        h = [(bl.pmax[b][d] - bl.pmin[b][d]) / (s[d] - 1) for d in range(3)]
+        w[bl.offsets[b]:bl.offsets[b] + s[0] * s[1] * s[2]] = h[0] * h[1] * h[2]
+    return w
+
+
+def constraint_drift(bl, u0, steps, dt, every=10):
+    """Evolve `u0` (not modified) for `steps` RK4 steps (captured RK4-step
+    graph: 4 ghost refreshes + 4 fused stages) in FP64 and in mixed precision;
+    volume-weighted L2 norms of the Hamiltonian and momentum constraints over
+    the interior nodes every `every` steps, always evaluated by an FP64 plan."""
     import torch
     import sympygr_amd as sgr
-    from sympygr_amd import data
-    dt = 0.25 * dx
-    npts = bl.interior_points()
     res = {"steps": steps, "dt": dt, "cfl": 0.25}
+    w = block_volume_weights(bl, u0.device)
+    vol = 0.0
+    for b in range(len(bl)):
+        s = bl.sz[b]
+        h = [(bl.pmax[b][d] - bl.pmin[b][d]) / (s[d] - 1) for d in range(3)]
+        vol += h[0] * h[1] * h[2] * (s[0] - 6) * (s[1] - 6) * (s[2] - 6)
     for prec in ("fp64", "mixed"):
-        u = data.fill_blocks(bl, backend="torch", device="cuda")
-        sa, sb, acc = torch.zeros_like(u), torch.zeros_like(u), torch.zeros_like(u)
-        c = torch.zeros((4, bl.total), dtype=torch.float64, device="cuda")
+        u = u0.clone()
+        sa, sb, acc = u0.clone(), u0.clone(), u0.clone()
+        c = torch.zeros((4, bl.total), dtype=torch.float64, device=u0.device)
         plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
-        cplan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64"))
+        old = os.environ.get("SGR_WS_GIB")
+        os.environ["SGR_WS_GIB"] = "8"  # the constraint plan runs in chunks beside the RK4 plan
+        try:
+            cplan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64"))
+        finally:
+            if old is None:
+                del os.environ["SGR_WS_GIB"]
+            else:
+                os.environ["SGR_WS_GIB"] = old
         graph = plan.rk4_graph(dt, u, sa, sb, acc)
         hist = []
 
         def norms():
             cplan.ghost_fill(u)
             cplan.constraints(u, c)
-            l2 = (c.pow(2).sum(dim=1) / npts).sqrt()
+            l2 = ((c * c * w).sum(dim=1) / vol).sqrt()
             return {"ham": l2[0].item(), "mom": (l2[1:] ** 2).sum().sqrt().item()}
         hist.append({"step": 0, **norms()})
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(1, steps + 1):
             graph.launch()
-            if s % 10 == 0:
+            if s % every == 0 or s == steps:
                 torch.cuda.synchronize()
                 hist.append({"step": s, **norms()})
+                log(f"drift {prec}: step {s} H {hist[-1]['ham']:.4e} M {hist[-1]['mom']:.4e}")
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         finite = bool(torch.isfinite(u).all().item())
         res[prec] = {"history": hist, "finite": finite, "wall_s": wall}
-        log(f"drift {prec}: H {hist[0]['ham']:.3e} -> {hist[-1]['ham']:.3e}, M {hist[0]['mom']:.3e} -> "
-            f"{hist[-1]['mom']:.3e}, finite={finite}, {wall:.1f} s")
         graph.close()
-        del plan, cplan, u, sa, sb, acc, c
+        del graph, plan, cplan, u, sa, sb, acc, c
         torch.cuda.empty_cache()
     hf, hm = res["fp64"]["history"][-1], res["mixed"]["history"][-1]
     res["ham_l2_rel_diff_mixed_vs_fp64"] = abs(hm["ham"] - hf["ham"]) / max(hf["ham"], 1e-300)
     res["mom_l2_rel_diff_mixed_vs_fp64"] = abs(hm["mom"] - hf["mom"]) / max(hf["mom"], 1e-300)
     return res
+
+
+def octree_section(args, hev):
+    """Batched uneven blocks: one RHS over all 3032 octree leaves (FP64 and
+    mixed, with the mixed error), the captured RK4 step on them (ghost
+    refresh across leaves of other sizes and levels), and the constraint
+    drift of FP64 against mixed."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    obl = data.octree_blocks()
+    t0 = time.perf_counter()
+    ou = data.fill_blocks(obl, backend="torch", device="cuda")
+    torch.cuda.synchronize()
+    opts = obl.interior_points()
+    log(f"octree: {len(obl)} uneven blocks, {opts} interior points, filled in {time.perf_counter() - t0:.1f} s")
+    res = {"workload": "3032 octree leaves over [-16,16]^3 refined toward the punctures, interior "
+                       "16/24/32/48 (+3 ghost), Brill-Lindquist, upwind, KO sigma 0.1",
+           "blocks": len(obl), "interior_points": opts, "padded_points": obl.total}
+    nst = max(2, args.steps // 4)
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        op = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+        outs[prec] = torch.zeros_like(ou)
+        w, _, _ = timed_steps(op, outs[prec], ou, nst, 1, hev)
+        res[prec] = {"value": opts * nst / w / 1e6, "chunks": op.info().chunks, "ms_per_step": 1e3 * w / nst}
+        log(f"octree {prec}: {res[prec]['value']:.1f} Mpts/s in {op.info().chunks} workspace chunks")
+        del op
+    err = normwise_err(outs["mixed"], outs["fp64"])
+    ref = outs["fp64"]
+    rms = ((outs["mixed"] - ref).pow(2).mean(dim=1).sqrt() / ref.pow(2).mean(dim=1).sqrt().clamp_min(1e-300))
+    res["mixed"]["max_normwise_rel_err_vs_fp64"] = err.max().item()
+    res["mixed"]["rms_over_fields_of_normwise_rel_err"] = err.pow(2).mean().sqrt().item()
+    res["mixed"]["per_field_normwise_rel_err"] = [float(f"{x:.3e}") for x in err.tolist()]
+    res["mixed"]["per_field_rms_rel_err"] = [float(f"{x:.3e}") for x in rms.tolist()]
+    res["mixed_speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    log(f"octree mixed error: max {err.max().item():.2e}")
+    del outs, ref, err, rms
+    torch.cuda.empty_cache()
+
+    hmin = min((obl.pmax[b][0] - obl.pmin[b][0]) / (obl.sz[b][0] - 1) for b in range(len(obl)))
+    dt = 0.25 * hmin
+    st = sgr.ghost_table_stats(obl)
+    res["ghost_table"] = st
+    res["rk4_step"] = {"dt": dt, "what": "captured graph: 4 ghost refreshes + 4 RHS with the RK4 update fused "
+                                         "into the output store"}
+    for prec in ("fp64", "mixed"):
+        plan = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+        x = ou.clone()
+        bufs = [ou.clone() for _ in range(3)]
+        g = plan.rk4_graph(dt, x, *bufs)
+        g.launch()
+        torch.cuda.synchronize()
+        k = 3
+        t0 = time.perf_counter()
+        for _ in range(k):
+            g.launch()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / k
+        t0 = time.perf_counter()
+        for _ in range(k):
+            plan.ghost_fill(x)
+        torch.cuda.synchronize()
+        gf = (time.perf_counter() - t0) / k
+        res["rk4_step"][prec] = {"ms_per_step": 1e3 * wall, "rhs_mpts_equiv": 4 * opts / wall / 1e6,
+                                 "ghost_fill_ms": 1e3 * gf}
+        log(f"octree rk4 {prec}: {1e3 * wall:.1f} ms per step, ghost refresh {1e3 * gf:.1f} ms")
+        g.close()
+        del g, plan, x, bufs
+        torch.cuda.empty_cache()
+
+    drift = None
+    if not args.no_drift:
+        drift = constraint_drift(obl, ou, args.drift_steps, dt)
+    del ou
+    torch.cuda.empty_cache()
+    return res, drift
 
 
 def cpu_baseline(bl, u, target_s=12.0):
@@ -210,13 +313,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-drift", action="store_true",
-                    help="skip the 100-step constraint-drift comparison of FP64 and mixed (config 3)")
-    ap.add_argument("--drift-steps", type=int, default=100)
+                    help="skip the constraint-drift comparison of FP64 and mixed on the octree")
+    ap.add_argument("--drift-steps", type=int, default=20)
     ap.add_argument("--no-octree", action="store_true",
-                    help="skip the batched uneven-octree measurement (config 2)")
+                    help="skip the batched uneven-octree measurements (RHS, RK4 step, drift) and the large block")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one 1 GiB device-to-device copy (reference point for HBM counters)")
+                    help="also run one 1 GiB device-to-device copy (k_hbm_copy, reference point for HBM counters)")
+    ap.add_argument("--counters", action="store_true",
+                    help="hardware-counter runs: inputs filled on the host and copied over, no work on the "
+                         "GPU but the FP64 RHS steps (and --copy-calibration)")
     args = ap.parse_args()
+    if args.counters:
+        args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the RHS path is single-GPU here); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -233,8 +341,12 @@ def main():
     n = 32
     bl = data.uniform_tiling(nb=8, n=n, stagger=0.5)
     log(f"filling {len(bl)} blocks ({bl.total} padded points per field)")
-    u = data.fill_blocks(bl, backend="torch", device="cuda")
-    out = torch.zeros_like(u)
+    if args.counters:
+        u = torch.from_numpy(data.fill_blocks(bl, backend="numpy")).cuda()
+        out = torch.empty_like(u)
+    else:
+        u = data.fill_blocks(bl, backend="torch", device="cuda")
+        out = torch.zeros_like(u)
     pts = bl.interior_points()
     hev = HipEvents()
 
@@ -243,7 +355,7 @@ def main():
     wall, kd, kp = timed_steps(plan, out, u, args.steps, args.warmup, hev)
     value = pts * args.steps / wall / 1e6
     log(f"fp64: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_derivs {kd:.3f} ms, k_pointwise {kp:.3f} ms")
-    ref_out = out.clone()
+    ref_out = out if args.counters else out.clone()
     del plan
 
     mixed = None
@@ -276,37 +388,11 @@ def main():
         "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                         "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
     }
-    drift = None
-    if not args.no_drift:
+    octree = drift = None
+    if not args.no_octree:
         del out, ref_out
         torch.cuda.empty_cache()
-        drift = constraint_drift(bl, args.drift_steps, 16.0 / (8 * n), hev)
-        out = torch.zeros_like(u)
-        ref_out = out
-
-    octree = None
-    if not args.no_octree:
-        del u, out, ref_out
-        torch.cuda.empty_cache()
-        obl = data.octree_blocks()
-        log(f"octree: {len(obl)} uneven blocks, {obl.interior_points()} interior points")
-        ou = data.fill_blocks(obl, backend="torch", device="cuda")
-        oout = torch.zeros_like(ou)
-        opts = obl.interior_points()
-        octree = {"workload": "3032 octree leaves over [-16,16]^3 refined toward the punctures, interior "
-                              "16/24/32/48 (+3 ghost), Brill-Lindquist, upwind, one RHS (RK4 stage) per step",
-                  "blocks": len(obl), "interior_points": opts}
-        for prec in ("fp64", "mixed"):
-            op = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
-            w, _, _ = timed_steps(op, oout, ou, max(2, args.steps // 4), 1, hev)
-            octree[prec] = {"value": opts * max(2, args.steps // 4) / w / 1e6, "chunks": op.info().chunks,
-                            "ms_per_step": 1e3 * w / max(2, args.steps // 4)}
-            log(f"octree {prec}: {octree[prec]['value']:.1f} Mpts/s in {op.info().chunks} workspace chunks")
-            del op
-        octree["mixed_speedup_vs_fp64"] = octree["mixed"]["value"] / octree["fp64"]["value"]
-        del ou, oout
-        torch.cuda.empty_cache()
-        u = data.fill_blocks(bl, backend="torch", device="cuda")
+        octree, drift = octree_section(args, hev)
 
     # a single large block (192^3 interior, 198^3 padded): the large-block,
     # single-launch shape (13824 tiles of one block)
@@ -330,10 +416,11 @@ def main():
         del lu, lout
 
     if args.copy_calibration:
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        import numpy as np
+        src = torch.from_numpy(np.ones(1 << 27)).cuda()
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)  # reads and writes exactly 1 GiB
+        sgr.hbm_copy(dst, src)  # k_hbm_copy: reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
     cpu = None

@@ -128,6 +128,12 @@ int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t st
  * error sgr_ghost_fill would return.  No reference counterpart. */
 int sgr_ghost_table_stats(const sgr_block *blocks, uint32_t nblocks, uint64_t stats[3]);
 
+/* Plain device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
+ * pointers) by one kernel, k_hbm_copy: reads and writes exactly `bytes`, the
+ * reference point that scales hardware byte counters.  No reference
+ * counterpart (measurement aid). */
+int sgr_hbm_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream);
+
 /* One classic-RK4 stage with the update fused into the RHS output store
  * (interior points only; a ghost refresh must precede the next stage):
  *   stage 0: k = RHS(u_stage); acc = u_n + dt/6 k; u_next = u_n + dt/2 k

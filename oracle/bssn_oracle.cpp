@@ -291,14 +291,14 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #define BSSN_U(f) in[f][pp]
 #define BSSN_D1(f, d) W.grad[f][d][pp]
 #define BSSN_D2(f, s) W.grad2[f][s][pp]
-#define BSSN_AD(f, d) W.agrad[f][d][pp]
+#define BSSN_ADV(f) ((in[6][pp] * W.agrad[f][0][pp] + in[7][pp] * W.agrad[f][1][pp]) + in[8][pp] * W.agrad[f][2][pp])
 #define BSSN_OUT(f, v) out[f][pp] = (v)
 #define BSSN_BARRIER()
 #include "gen/bssn_algebra.inc"
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
-#undef BSSN_AD
+#undef BSSN_ADV
 #undef BSSN_OUT
 #undef BSSN_BARRIER
             }

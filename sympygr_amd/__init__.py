@@ -92,6 +92,7 @@ _SIGS = {
                                       ctypes.POINTER(ctypes.c_void_p)]),
     "sgr_constraints": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_ghost_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_hbm_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
     "sgr_ghost_table_stats": (ctypes.c_int, [ctypes.POINTER(SgrBlock), ctypes.c_uint32,
                                              ctypes.POINTER(ctypes.c_uint64)]),
     "sgr_rk4_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(P_D),
@@ -158,6 +159,18 @@ def block_array(bl):
             arr[b].pmax[d] = bl.pmax[b][d]
         arr[b].bflag = bl.bflag[b]
     return arr
+
+
+def hbm_copy(dst, src, stream=None):
+    """Copy device tensor src into dst (same byte size) with the library's
+    plain copy kernel."""
+    import torch
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("size mismatch")
+    s = stream if stream is not None else torch.cuda.current_stream(src.device).cuda_stream
+    check(lib().sgr_hbm_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                             ctypes.c_void_p(s)))
 
 
 def ghost_table_stats(bl):
@@ -292,9 +305,11 @@ class Graph:
         check(lib().sgr_graph_launch(self._g, ctypes.c_void_p(s)))
 
     def close(self):
+        """Destroy the graph and release the plan and buffers it kept alive."""
         if self._g:
             lib().sgr_graph_destroy(self._g)
             self._g = ctypes.c_void_p()
+        self._keep = None
 
     def __del__(self):
         try:

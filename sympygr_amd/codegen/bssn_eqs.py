@@ -18,8 +18,10 @@ arithmetic is grouped:
 Variable names follow the reference: fields ``alpha chi K Gt0..2 beta0..2
 B0..2 gt0..5 At0..5`` (bssn/include/grDef.h:31, symmetric order
 xx,xy,xz,yy,yz,zz as GR/dendro.py:86-98), first derivatives
-``grad_<d>_<f>``, second derivatives ``grad2_<i>_<j>_<f>`` with i<=j and
-advective derivatives ``agrad_<d>_<f>`` (bssn/scripts/bssnrhs_derivs*.h).
+``grad_<d>_<f>``, second derivatives ``grad2_<i>_<j>_<f>`` with i<=j
+(bssn/scripts/bssnrhs_derivs*.h).  The advective derivatives ``agrad_<d>_<f>``
+appear in the reference system only as beta^d agrad_d f; that contraction is
+the input ``adv_<f>``, one per field.
 """
 from __future__ import annotations
 
@@ -76,10 +78,13 @@ class Inputs:
         a, b = min(i, j), max(i, j)
         return sp.Symbol(f"grad2_{a}_{b}_{f}")
 
-    # advective d_i f (upwinded by sign(beta^i), or = d1 in centred mode)
+    # advection of field f: beta^i ad_i f, the only form in which the
+    # advective derivatives agrad_i_f (upwinded by sign(beta^i), or = d1 in
+    # centred mode) enter the system; evaluated beside the derivatives as
+    # (beta0 ad_0 f + beta1 ad_1 f) + beta2 ad_2 f
     @staticmethod
-    def ad(f: str, i: int) -> sp.Symbol:
-        return sp.Symbol(f"agrad_{i}_{f}")
+    def adv(f: str) -> sp.Symbol:
+        return sp.Symbol(f"adv_{f}")
 
 
 def _sym3(prefix: str):
@@ -103,7 +108,7 @@ def bssn_rhs_stages() -> Stages:
     I = Inputs()
     u = I.u
     P = I.params
-    d1, d2, ad = I.d1, I.d2, I.ad
+    d1, d2, adv = I.d1, I.d2, I.adv
     S = Stages()
 
     alpha, chi, K = u["alpha"], u["chi"], u["K"]
@@ -240,16 +245,16 @@ def bssn_rhs_stages() -> Stages:
 
     # ---- the RHS, GR/bssn.py:67-100
     out = {}
-    out["a_rhs"] = P["lambda0"] * sum(b[i] * ad("alpha", i) for i in E3) - 2 * alpha * K
+    out["a_rhs"] = P["lambda0"] * adv("alpha") - 2 * alpha * K
     for i in E3:
         out[f"b_rhs{i}"] = sp.Rational(3, 4) * (P["lambda_f0"] + P["lambda_f1"] * alpha) * B[i] \
-            + P["lambda1"] * sum(b[j] * ad(bn[i], j) for j in E3)
+            + P["lambda1"] * adv(bn[i])
     w = -sp.Rational(2, 3)
     for (i, j) in SYM:
-        lie = sum(b[k] * ad(gtn[i][j], k) + gt[i][k] * d1(bn[k], j) + gt[k][j] * d1(bn[k], i) for k in E3) \
+        lie = adv(gtn[i][j]) + sum(gt[i][k] * d1(bn[k], j) + gt[k][j] * d1(bn[k], i) for k in E3) \
             + w * gt[i][j] * divb
         out[f"gt_rhs{i}{j}"] = lie - 2 * alpha * At[i][j]
-    out["chi_rhs"] = sum(b[i] * ad("chi", i) for i in E3) + w * chi * divb + sp.Rational(2, 3) * chi * alpha * K
+    out["chi_rhs"] = adv("chi") + w * chi * divb + sp.Rational(2, 3) * chi * alpha * K
 
     # trace-free part of (alpha R - DDa), GR/dendro.py:321-336
     X = [[None] * 3 for _ in E3]
@@ -258,20 +263,20 @@ def bssn_rhs_stages() -> Stages:
     trX = sp.Symbol("trX")
     S.stage("trX", [(trX, sum(igt[k][l] * X[k][l] for (k, l) in E33))])
     for (i, j) in SYM:
-        lie = sum(b[k] * ad(Atn[i][j], k) + At[i][k] * d1(bn[k], j) + At[k][j] * d1(bn[k], i) for k in E3) \
+        lie = adv(Atn[i][j]) + sum(At[i][k] * d1(bn[k], j) + At[k][j] * d1(bn[k], i) for k in E3) \
             + w * At[i][j] * divb
         out[f"At_rhs{i}{j}"] = lie + chi * (X[i][j] - gt[i][j] * trX * sp.Rational(1, 3)) \
             + alpha * (K * At[i][j] - 2 * AA[i][j])
 
     # K: lie - laplacian(alpha) + alpha (K^2/3 + A_ij A^ij); laplacian uses
     # the inverse physical metric chi igt (GR/dendro.py:259-275)
-    out["K_rhs"] = sum(b[i] * ad("K", i) for i in E3) \
+    out["K_rhs"] = adv("K") \
         - chi * sum(igt[i][j] * DDa[i][j] for (i, j) in E33) \
         + alpha * (K * K * sp.Rational(1, 3) + sum(At[i][j] * AtU[i][j] for (i, j) in E33))
 
     Gt_rhs = []
     for i in E3:
-        e = sum(b[j] * ad(Gtn[i], j) for j in E3)
+        e = adv(Gtn[i])
         e -= sum(CalGt[j] * d1(bn[i], j) for j in E3)
         e += sp.Rational(2, 3) * CalGt[i] * divb
         e += sum(igt[j][k] * d2(bn[i], j, k) + igt[i][j] * d2(bn[k], j, k) * sp.Rational(1, 3) for (j, k) in E33)
@@ -283,8 +288,8 @@ def bssn_rhs_stages() -> Stages:
     S.stage("Gt_rhs", [(Gts[i], Gt_rhs[i]) for i in E3])
     for i in E3:
         out[f"Gt_rhs{i}"] = Gts[i]
-        out[f"B_rhs{i}"] = Gts[i] - P["eta"] * B[i] + P["lambda2"] * sum(b[j] * ad(Bn[i], j) for j in E3) \
-            - P["lambda3"] * sum(b[j] * ad(Gtn[i], j) for j in E3)
+        out[f"B_rhs{i}"] = Gts[i] - P["eta"] * B[i] + P["lambda2"] * adv(Bn[i]) \
+            - P["lambda3"] * adv(Gtn[i])
 
     name_map = {"gt_rhs00": "gt_rhs00", "gt_rhs01": "gt_rhs01", "gt_rhs02": "gt_rhs02",
                 "gt_rhs11": "gt_rhs11", "gt_rhs12": "gt_rhs12", "gt_rhs22": "gt_rhs22"}

@@ -13,7 +13,7 @@ them:
     BSSN_U(f)          field f at this point
     BSSN_D1(f, d)      first derivative of field f along d
     BSSN_D2(f, s)      second derivative of field f, symmetric slot s
-    BSSN_AD(f, d)      advective derivative of field f along d
+    BSSN_ADV(f)        advection of field f: beta^d times its advective derivative
     BSSN_OUT(f, v)     store RHS of field f
 
 Parameters (lambda0..3, lambda_f0..1, eta) are plain variables of type R
@@ -31,7 +31,7 @@ from .bssn_eqs import FIELD_INDEX, SYM
 
 _D1 = re.compile(r"^grad_(\d)_(\w+)$")
 _D2 = re.compile(r"^grad2_(\d)_(\d)_(\w+)$")
-_AD = re.compile(r"^agrad_(\d)_(\w+)$")
+_ADV = re.compile(r"^adv_(\w+)$")
 
 
 def classify_input(name: str):
@@ -44,9 +44,9 @@ def classify_input(name: str):
     m = _D1.match(name)
     if m:
         return ("D1", m.group(2), int(m.group(1)))
-    m = _AD.match(name)
-    if m:
-        return ("AD", m.group(2), int(m.group(1)))
+    m = _ADV.match(name)
+    if m and m.group(1) in FIELD_INDEX:
+        return ("ADV", m.group(1), None)
     return None
 
 
@@ -139,6 +139,8 @@ def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=Fa
             fi = FIELD_INDEX[f]
             if kind == "U":
                 lines.append(f"{indent}const R {fs.name} = BSSN_U({fi});")
+            elif kind == "ADV":
+                lines.append(f"{indent}const R {fs.name} = BSSN_ADV({fi});")
             else:
                 lines.append(f"{indent}const R {fs.name} = BSSN_{kind}({fi}, {idx});")
 

@@ -116,6 +116,13 @@ __global__ void k_check_finite(FieldPtrs in, const BlockDesc *blocks, uint32_t n
     }
 }
 
+// Plain HBM copy, 16 bytes per lane, grid-stride (a reference point of known
+// traffic for hardware byte counters).
+__global__ __launch_bounds__(256) void k_hbm_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) dst[i] = src[i];
+}
+
 }  // namespace
 
 struct GhostTable {
@@ -353,7 +360,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     }
     const bool fusedv = p->variant == Variant::Fused;
     const int tdim[3] = {fusedv ? FUSED_TX : T, fusedv ? FUSED_TY : T, fusedv ? FUSED_TZ : T};
-    p->nslots = fusedv ? 0 : (params->adv_mode == SGR_ADV_UPWIND ? NSLOT_UPWIND : NSLOT_CENTRED);
+    p->nslots = fusedv ? 0 : NSLOT;
     std::vector<uint32_t> tile_block;
     uint64_t ws_base = 0;
     for (uint32_t b = 0; b < nblocks; b++) {
@@ -604,6 +611,20 @@ int sgr_ghost_table_stats(const sgr_block *blocks, uint32_t nblocks, uint64_t st
     stats[1] = T.nodes;
     stats[2] = T.interp_nodes;
     if (!T.complete) return fail(SGR_E_UNSUPPORTED, T.msg);
+    return SGR_OK;
+}
+
+int sgr_hbm_copy(void *dst, const void *src, uint64_t bytes, hipStream_t stream)
+{
+    if (!dst || !src) return fail(SGR_E_INVALID, "null argument");
+    if (bytes % 16 || (uintptr_t)dst % 16 || (uintptr_t)src % 16)
+        return fail(SGR_E_INVALID, "pointers and size must be multiples of 16 bytes");
+    if (bytes == 0) return SGR_OK;
+    const uint64_t n = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 256ull * 64);
+    hipLaunchKernelGGL(k_hbm_copy, dim3(grid), dim3(256), 0, stream, (double2 *)dst, (const double2 *)src, n);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "copy launch");
     return SGR_OK;
 }
 

@@ -27,16 +27,17 @@ __host__ __device__ constexpr int d2field(int q)
 // Derivative workspace slots, one plane of NP points each:
 //   [0,72)     first derivative of field f along d: f*3+d
 //   [72,138)   second derivatives, 72 + q*6 + s (s: xx xy xz yy yz zz)
-//   [138,210)  advective derivatives (upwind mode only), 138 + f*3+d
+//   [138,162)  advection of field f, 138 + f:
+//              (beta^0 ad_0 f + beta^1 ad_1 f) + beta^2 ad_2 f, with ad the
+//              upwinded stencil (upwind mode) or the centred one
 constexpr int SLOT_D1 = 0;
 constexpr int SLOT_D2 = 72;
-constexpr int SLOT_AD = 138;
-constexpr int NSLOT_CENTRED = 138;
-constexpr int NSLOT_UPWIND = 210;
+constexpr int SLOT_ADV = 138;
+constexpr int NSLOT = 162;
 
 __host__ __device__ constexpr int slot_d1(int f, int d) { return SLOT_D1 + f * 3 + d; }
 __host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + d2q(f) * 6 + s; }
-__host__ __device__ constexpr int slot_ad(int f, int d) { return SLOT_AD + f * 3 + d; }
+__host__ __device__ constexpr int slot_adv(int f) { return SLOT_ADV + f; }
 
 // Sommerfeld falloff / asymptotic value per field (bssn/src/rhs.cpp:261-320).
 __host__ __device__ constexpr double bc_falloff(int f)

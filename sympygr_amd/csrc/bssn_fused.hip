@@ -62,6 +62,25 @@ __device__ __forceinline__ double radius_f(double x, double y, double z)
     return sqrt(x * x + y * y + z * z);
 }
 
+// advection of the current field: (beta^0 ad_0 + beta^1 ad_1) + beta^2 ad_2
+__device__ inline double fused_adv(const double *cu, int i, int j, int k, int n0, int n1, int n2, bool lo0, bool hi0,
+                                   bool lo1, bool hi1, bool lo2, bool hi2, const AxisConst &a0, const AxisConst &a1,
+                                   const AxisConst &a2, double bv0, double bv1, double bv2, bool upwind)
+{
+    SGR_EXACT
+    double q0, q1, q2;
+    if (upwind) {
+        q0 = adv(cu, 1, i, n0, lo0, hi0, bv0 > 0.0 ? 1.0 : 0.0, a0);
+        q1 = adv(cu, fused::HX, j, n1, lo1, hi1, bv1 > 0.0 ? 1.0 : 0.0, a1);
+        q2 = adv(cu, fused::HX * fused::HY, k, n2, lo2, hi2, bv2 > 0.0 ? 1.0 : 0.0, a2);
+    } else {
+        q0 = d1(cu, 1, i, n0, lo0, hi0, a0);
+        q1 = d1(cu, fused::HX, j, n1, lo1, hi1, a1);
+        q2 = d1(cu, fused::HX * fused::HY, k, n2, lo2, hi2, a2);
+    }
+    return (bv0 * q0 + bv1 * q1) + bv2 * q2;
+}
+
 template <typename R, bool UPWIND>
 __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
                                                const uint32_t *__restrict__ tile_block, Params P)
@@ -107,13 +126,8 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
     const R lambda_f0 = (R)P.lambda_f[0], lambda_f1 = (R)P.lambda_f[1];
     const R eta = (R)eta_d;
     const double sigma = P.ko_sigma;
-    // upwind direction per axis: beta^d > 0 (bssn/src/derivs.cpp:518)
-    bool bpos0 = false, bpos1 = false, bpos2 = false;
-    if (UPWIND) {
-        bpos0 = in.p[6][pp] > 0.0;
-        bpos1 = in.p[7][pp] > 0.0;
-        bpos2 = in.p[8][pp] > 0.0;
-    }
+    // shift at the point (advection; upwind direction beta^d > 0, bssn/src/derivs.cpp:518)
+    const double bv0 = valid ? in.p[6][pp] : 0.0, bv1 = valid ? in.p[7][pp] : 0.0, bv2 = valid ? in.p[8][pp] : 0.0;
 
     // halo element e of the tile -> global offset in the block (or -1)
     auto halo_src = [&](int e) -> int32_t {
@@ -174,13 +188,8 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
     ((R)((D_) == 0 ? d1(&cur[c], 1, i, n0, lo0, hi0, a0)                                                      \
                    : (D_) == 1 ? d1(&cur[c], HX, j, n1, lo1, hi1, a1) : d1(&cur[c], HX * HY, k, n2, lo2, hi2, a2)))
 
-#define BSSN_AD(F_, D_)                                                                                        \
-    ((R)(!UPWIND ? ((D_) == 0 ? d1(&cur[c], 1, i, n0, lo0, hi0, a0)                                           \
-                              : (D_) == 1 ? d1(&cur[c], HX, j, n1, lo1, hi1, a1)                              \
-                                          : d1(&cur[c], HX * HY, k, n2, lo2, hi2, a2))                        \
-                 : ((D_) == 0 ? adv(&cur[c], 1, i, n0, lo0, hi0, bpos0 ? 1.0 : 0.0, a0)                       \
-                              : (D_) == 1 ? adv(&cur[c], HX, j, n1, lo1, hi1, bpos1 ? 1.0 : 0.0, a1)          \
-                                          : adv(&cur[c], HX * HY, k, n2, lo2, hi2, bpos2 ? 1.0 : 0.0, a2))))
+#define BSSN_ADV(F_)                                                                                           \
+    ((R)fused_adv(&cur[c], i, j, k, n0, n1, n2, lo0, hi0, lo1, hi1, lo2, hi2, a0, a1, a2, bv0, bv1, bv2, UPWIND))
 
 #define BSSN_D2(F_, S_)                                                                                        \
     ((R)((S_) == 0   ? d2(&cur[c], 1, i, n0, lo0, hi0, a0)                                                    \
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
 #undef BSSN_STAGE_BEGIN
 #undef BSSN_STAGE_END
 #undef BSSN_D1
-#undef BSSN_AD
+#undef BSSN_ADV
 #undef BSSN_D2
 #undef BSSN_FIELD_DONE
 #undef BSSN_OUT

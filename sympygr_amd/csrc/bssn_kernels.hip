@@ -60,7 +60,7 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
 }
 
 // Fields whose first derivatives the pointwise algebra never reads (B, At):
-// with upwinded advection their first derivatives are only needed by the
+// apart from their advection (stored contracted) they are only needed by the
 // Sommerfeld condition, i.e. on tiles that touch a flagged face.
 __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && f <= 11) || f >= 18; }
 
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
     bool valid[2];
     int pk[2];
     uint64_t wp[2];
-    bool bpos[2][3];
+    double bv[2][3];  // shift at the point
     // slot-major workspace with a padded plane stride SS (points); this tile's
     // 512 points are contiguous within every plane
     W *__restrict__ wsb = ws + (uint64_t)ct * TILE_PTS;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
         wp[p] = (uint64_t)(lx + T * (ly + T * (lz + 4 * p)));
         const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) bpos[p][d] = (UPWIND && valid[p]) ? in.p[6 + d][gp] > 0.0 : false;
+        for (int d = 0; d < 3; d++) bv[p][d] = valid[p] ? in.p[6 + d][gp] : 0.0;
     }
     // halo element -> offset in the block (or -1 outside the padded block)
     int32_t src[NPF];
@@ -133,20 +133,31 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
 #pragma unroll
             for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
         }
-        const bool need_d1 = all_d1 || !UPWIND || !d1_only_for_faces(f) || face_tile;
+        const bool need_d1 = all_d1 || !d1_only_for_faces(f) || face_tile;
 #pragma unroll
         for (int p = 0; p < 2; p++) {
             if (!valid[p]) continue;
             const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-            if (need_d1) {
-                wsb[slot_d1(f, 0) * SS + wp[p]] = (W)d1(c, 1, pi, n0, lo0, hi0, a0);
-                wsb[slot_d1(f, 1) * SS + wp[p]] = (W)d1(c, TH, pj, n1, lo1, hi1, a1);
-                wsb[slot_d1(f, 2) * SS + wp[p]] = (W)d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+            if (need_d1 || (!UPWIND && !all_d1)) {
+                const double g0 = d1(c, 1, pi, n0, lo0, hi0, a0);
+                const double g1 = d1(c, TH, pj, n1, lo1, hi1, a1);
+                const double g2 = d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                if (need_d1) {
+                    wsb[slot_d1(f, 0) * SS + wp[p]] = (W)g0;
+                    wsb[slot_d1(f, 1) * SS + wp[p]] = (W)g1;
+                    wsb[slot_d1(f, 2) * SS + wp[p]] = (W)g2;
+                }
+                if (!UPWIND && !all_d1) {
+                    SGR_EXACT
+                    wsb[slot_adv(f) * SS + wp[p]] = (W)((bv[p][0] * g0 + bv[p][1] * g1) + bv[p][2] * g2);
+                }
             }
-            if (UPWIND) {
-                wsb[slot_ad(f, 0) * SS + wp[p]] = (W)adv(c, 1, pi, n0, lo0, hi0, bpos[p][0] ? 1.0 : 0.0, a0);
-                wsb[slot_ad(f, 1) * SS + wp[p]] = (W)adv(c, TH, pj, n1, lo1, hi1, bpos[p][1] ? 1.0 : 0.0, a1);
-                wsb[slot_ad(f, 2) * SS + wp[p]] = (W)adv(c, TH * TH, pk[p], n2, lo2, hi2, bpos[p][2] ? 1.0 : 0.0, a2);
+            if (UPWIND && !all_d1) {
+                SGR_EXACT
+                const double q0 = adv(c, 1, pi, n0, lo0, hi0, bv[p][0] > 0.0 ? 1.0 : 0.0, a0);
+                const double q1 = adv(c, TH, pj, n1, lo1, hi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
+                const double q2 = adv(c, TH * TH, pk[p], n2, lo2, hi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
+                wsb[slot_adv(f) * SS + wp[p]] = (W)((bv[p][0] * q0 + bv[p][1] * q1) + bv[p][2] * q2);
             }
             {
                 SGR_EXACT

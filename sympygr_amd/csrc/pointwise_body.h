@@ -5,7 +5,7 @@
 // applies the Sommerfeld condition on flagged faces and adds sigma * KO in
 // double.
 template <typename R, typename W, bool UPWIND, bool RK>
-__global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
+__global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, const BlockDesc *__restrict__ blocks,
                                                    const uint32_t *__restrict__ tile_block,
                                                    const W *__restrict__ ws, const double *__restrict__ wsK,
                                                    uint64_t SS, Params P, RkArgs rk, uint32_t tile0)
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
 #define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
-#define BSSN_AD(F_, D_) ((R)wsb[(UPWIND ? slot_ad(F_, D_) : slot_d1(F_, D_)) * SS])
+#define BSSN_ADV(F_) ((R)wsb[slot_adv(F_) * SS])
 #define BSSN_OUT(F_, V_)                                                                                      \
     do {                                                                                                    \
         double val_ = (double)(V_);                                                                         \
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void SGR_PW_NAME(FieldPtrs in, OutPtrs out, co
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
-#undef BSSN_AD
+#undef BSSN_ADV
 #undef BSSN_OUT
 }
 

@@ -6,7 +6,7 @@ import os
 import sympy as sp
 
 from sympygr_amd.codegen import __main__ as gen
-from sympygr_amd.codegen.bssn_eqs import bssn_rhs_stages
+from sympygr_amd.codegen.bssn_eqs import FIELDS, bssn_rhs_stages
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -20,16 +20,19 @@ def test_committed_algebra_is_up_to_date():
 
 
 def test_input_set_matches_reference_derivative_lists():
-    """45 first derivatives, 66 second (11 fields x 6), 72 advective, 21 field
-    values (Gt is only differentiated), as bssn/scripts/bssnrhs_*.h allocate."""
+    """45 first derivatives, 66 second (11 fields x 6), as
+    bssn/scripts/bssnrhs_*.h allocate; the 72 advective derivatives enter
+    only as one contraction beta^d agrad_d f per field (24); 18 field values
+    (Gt is only differentiated, the shift only advects)."""
     E = gen.generate(write=False)
     kinds = {}
     for k, f, i in E.inputs:
         kinds.setdefault(k, set()).add((f, i))
     assert len(kinds["D1"]) == 45
     assert len(kinds["D2"]) == 66
-    assert len(kinds["AD"]) == 72
-    assert len(kinds["U"]) == 21
+    assert len(kinds["ADV"]) == 24
+    assert len(kinds["U"]) == 18
+    assert {f for f, _ in kinds["ADV"]} == set(FIELDS)
     d2fields = {f for f, _ in kinds["D2"]}
     assert d2fields == {"alpha", "chi", "beta0", "beta1", "beta2", "gt0", "gt1", "gt2", "gt3", "gt4", "gt5"}
 
