@@ -48,6 +48,24 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
 
+def measured_traffic(points):
+    """HBM bytes of one RHS pass on this workload as last measured with
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (per dispatch, summed over the
+    pass's kernels, each counter scaled by its reading for a 1 GiB k_hbm_copy);
+    None when no measurement for this workload is present."""
+    path = os.path.join(ROOT, "profiles", "pmc_bssn_rhs_512x32cubed_fp64.json")
+    try:
+        with open(path) as f:
+            J = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if J.get("interior_points") != points:
+        return None, None
+    per = {k.split("<")[0]: round(v["read_bytes"] + v["write_bytes"]) for k, v in J["kernels"].items()}
+    return J["rhs_pass_bytes"], {"per_kernel_bytes": per, "measured_at": J.get("git"),
+                                 "calibration": J["calibration"]}
+
+
 def log(msg):
     sys.stderr.write(f"[bench] {msg}\n")
     sys.stderr.flush()
@@ -379,9 +397,10 @@ def main():
     achieved_gbs = abytes / (rhs_ms * 1e-3) / 1e9
     achieved_tfs = FLOPS_UPWIND * pts / (rhs_ms * 1e-3) / 1e12
     dominant = "k_pointwise" if kp >= kd else "k_derivs"
+    traffic, traffic_detail = measured_traffic(pts)
     roofline = {
         "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+        "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": traffic_detail,
         "kernel": "RHS pass = k_derivs + k_pointwise (event-timed per launch)",
         "dominant_kernel": dominant, "kernel_ms": {"k_derivs": kd, "k_pointwise": kp},
         "algorithmic_bytes_per_point": alg_bytes_per_point(n),
