@@ -256,6 +256,26 @@ void sweep_all(Workspace &W, const double *const *in, const double *h, const Gri
             for (int a = 0; a < 3; a++) advective(W.agrad[f][a], in[f], h[a], g, in[6 + a], bflag, a);
 }
 
+// igt^{lm} d_l d_m of a conformal-metric component at one point: inverse
+// metric by adjugate / determinant of gt0..5 (xx xy xz yy yz zz), then
+// sum_s w_s igt_s grad2_s, w = (1,2,2,1,2,1), in slot order.  This is the
+// first term of the conformal Ricci tensor of GR/dendro.py:495-499 (the
+// second derivatives of gt enter the system only in this contraction).
+double metric_laplacian(const double *const *in, double *const *grad2, std::ptrdiff_t pp)
+{
+    const double g0 = in[12][pp], g1 = in[13][pp], g2 = in[14][pp], g3 = in[15][pp], g4 = in[16][pp], g5 = in[17][pp];
+    const double c0 = g3 * g5 - g4 * g4;
+    const double c1 = g2 * g4 - g1 * g5;
+    const double c2 = g1 * g4 - g2 * g3;
+    const double c3 = g0 * g5 - g2 * g2;
+    const double c4 = g1 * g2 - g0 * g4;
+    const double c5 = g0 * g3 - g1 * g1;
+    const double idet = 1.0 / (g0 * c0 + g1 * c1 + g2 * c2);
+    const double ig[6] = {c0 * idet, c1 * idet, c2 * idet, c3 * idet, c4 * idet, c5 * idet};
+    return ((((ig[0] * grad2[0][pp] + 2.0 * (ig[1] * grad2[1][pp])) + 2.0 * (ig[2] * grad2[2][pp])) +
+              ig[3] * grad2[3][pp]) + 2.0 * (ig[4] * grad2[4][pp])) + ig[5] * grad2[5][pp];
+}
+
 int rhs_block(double **outv, const double **inv, uint64_t offset, const double *pmin, const double *pmax,
               const uint32_t *sz, uint32_t bflag, const oracle_params *P, Workspace &W)
 {
@@ -292,6 +312,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #define BSSN_D1(f, d) W.grad[f][d][pp]
 #define BSSN_D2(f, s) W.grad2[f][s][pp]
 #define BSSN_ADV(f) ((in[6][pp] * W.agrad[f][0][pp] + in[7][pp] * W.agrad[f][1][pp]) + in[8][pp] * W.agrad[f][2][pp])
+#define BSSN_LAP(f) metric_laplacian(in, W.grad2[f], pp)
 #define BSSN_OUT(f, v) out[f][pp] = (v)
 #define BSSN_BARRIER()
 #include "gen/bssn_algebra.inc"
@@ -299,6 +320,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #undef BSSN_D1
 #undef BSSN_D2
 #undef BSSN_ADV
+#undef BSSN_LAP
 #undef BSSN_OUT
 #undef BSSN_BARRIER
             }
@@ -392,12 +414,14 @@ extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t of
 #define BSSN_U(f) in[f][pp]
 #define BSSN_D1(f, d) W.grad[f][d][pp]
 #define BSSN_D2(f, s) W.grad2[f][s][pp]
+#define BSSN_LAP(f) metric_laplacian(in, W.grad2[f], pp)
 #define BSSN_OUT(c, v) out[c][pp] = (v)
 #define BSSN_BARRIER()
 #include "gen/bssn_constraints.inc"
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
+#undef BSSN_LAP
 #undef BSSN_OUT
 #undef BSSN_BARRIER
             }

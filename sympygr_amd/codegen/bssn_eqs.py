@@ -21,7 +21,8 @@ xx,xy,xz,yy,yz,zz as GR/dendro.py:86-98), first derivatives
 ``grad_<d>_<f>``, second derivatives ``grad2_<i>_<j>_<f>`` with i<=j
 (bssn/scripts/bssnrhs_derivs*.h).  The advective derivatives ``agrad_<d>_<f>``
 appear in the reference system only as beta^d agrad_d f; that contraction is
-the input ``adv_<f>``, one per field.
+the input ``adv_<f>``, one per field.  The second derivatives of gt_ij appear
+only as igt^{lm} grad2_l_m_gt_ij, the input ``lap_<gt_ij>``.
 """
 from __future__ import annotations
 
@@ -85,6 +86,14 @@ class Inputs:
     @staticmethod
     def adv(f: str) -> sp.Symbol:
         return sp.Symbol(f"adv_{f}")
+
+    # igt^{lm} d_l d_m gt_ij: the only form in which the second derivatives of
+    # the conformal metric enter (the first term of the conformal Ricci
+    # tensor), evaluated beside the derivatives with the inverse metric of the
+    # point as sum over slots s of w_s igt_s d2_s gt_ij, w = (1,2,2,1,2,1)
+    @staticmethod
+    def lap(f: str) -> sp.Symbol:
+        return sp.Symbol(f"lap_{f}")
 
 
 def _sym3(prefix: str):
@@ -199,7 +208,7 @@ def bssn_rhs_stages() -> Stages:
     pairs = []
     for (i, j) in SYM:
         s = sp.Symbol(f"Rt{sym_slot(i, j)}")
-        e = -half * sum(igt[l][m] * d2(gtn[i][j], l, m) for (l, m) in E33)
+        e = -half * I.lap(gtn[i][j])
         e += half * sum(gt[k][i] * d1(Gtn[k], j) + gt[k][j] * d1(Gtn[k], i) for k in E3)
         e += half * sum(CalGt[k] * (C1[i][j][k] + C1[j][i][k]) for k in E3)
         e += sum(C2[k][l][i] * C1u[j][k][l] + C2[k][l][j] * C1u[i][k][l] for k in E3 for l in E3)

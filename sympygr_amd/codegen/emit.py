@@ -14,6 +14,7 @@ them:
     BSSN_D1(f, d)      first derivative of field f along d
     BSSN_D2(f, s)      second derivative of field f, symmetric slot s
     BSSN_ADV(f)        advection of field f: beta^d times its advective derivative
+    BSSN_LAP(f)        igt^{lm} d_l d_m of field f (conformal metric components)
     BSSN_OUT(f, v)     store RHS of field f
 
 Parameters (lambda0..3, lambda_f0..1, eta) are plain variables of type R
@@ -32,6 +33,7 @@ from .bssn_eqs import FIELD_INDEX, SYM
 _D1 = re.compile(r"^grad_(\d)_(\w+)$")
 _D2 = re.compile(r"^grad2_(\d)_(\d)_(\w+)$")
 _ADV = re.compile(r"^adv_(\w+)$")
+_LAP = re.compile(r"^lap_(\w+)$")
 
 
 def classify_input(name: str):
@@ -47,6 +49,9 @@ def classify_input(name: str):
     m = _ADV.match(name)
     if m and m.group(1) in FIELD_INDEX:
         return ("ADV", m.group(1), None)
+    m = _LAP.match(name)
+    if m and m.group(1) in FIELD_INDEX:
+        return ("LAP", m.group(1), None)
     return None
 
 
@@ -139,8 +144,8 @@ def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=Fa
             fi = FIELD_INDEX[f]
             if kind == "U":
                 lines.append(f"{indent}const R {fs.name} = BSSN_U({fi});")
-            elif kind == "ADV":
-                lines.append(f"{indent}const R {fs.name} = BSSN_ADV({fi});")
+            elif kind in ("ADV", "LAP"):
+                lines.append(f"{indent}const R {fs.name} = BSSN_{kind}({fi});")
             else:
                 lines.append(f"{indent}const R {fs.name} = BSSN_{kind}({fi}, {idx});")
 

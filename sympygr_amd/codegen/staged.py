@@ -16,7 +16,7 @@ Macros the including kernel defines (all inside one function body):
                                  staged next, -1 after the last)
     BSSN_STAGE_END(f, s, fn)     end of stage s
     BSSN_U(f)                    value of field f at the point (any time)
-    BSSN_D1(f, d) BSSN_D2(f, s) BSSN_ADV(f)   derivatives / advection of the current field
+    BSSN_D1(f, d) BSSN_D2(f, s) BSSN_ADV(f) BSSN_LAP(f)   derivatives of the current field
     BSSN_FIELD_DONE(f)           per-field work that needs only field f
                                  (Kreiss-Oliger, boundary value)
     BSSN_OUT(f, v)               final value of RHS f
@@ -242,7 +242,7 @@ def emit_staged(S, field_order=None, indent="    ", outputs=None) -> StagedEmitt
         fnext = FIELD_INDEX[order[s + 1]] if s + 1 < len(order) else -1
         lines.append(f"{indent}BSSN_STAGE_BEGIN({fi}, {s}, {1 if f in D2_FIELDS else 0}, {fnext});")
         for x, (kind, fld, idx) in sorted(deriv_inputs[s], key=lambda t: t[0].name):
-            args = f"{fi}" if kind == "ADV" else f"{fi}, {idx}"
+            args = f"{fi}" if kind in ("ADV", "LAP") else f"{fi}, {idx}"
             lines.append(f"{indent}const R {x.name} = BSSN_{kind}({args});")
             step[0] += 1
             pos_def[x] = step[0]

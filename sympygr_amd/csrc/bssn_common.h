@@ -26,18 +26,51 @@ __host__ __device__ constexpr int d2field(int q)
 
 // Derivative workspace slots, one plane of NP points each:
 //   [0,72)     first derivative of field f along d: f*3+d
-//   [72,138)   second derivatives, 72 + q*6 + s (s: xx xy xz yy yz zz)
-//   [138,162)  advection of field f, 138 + f:
+//   [72,102)   second derivatives of alpha, beta0..2, chi: 72 + q*6 + s
+//              (q = d2q(f) in 0..4; s: xx xy xz yy yz zz)
+//   [102,108)  igt^{lm} d_l d_m gt_f for the six conformal-metric fields,
+//              102 + (f - 12), igt the inverse conformal metric of the point
+//   [108,132)  advection of field f, 108 + f:
 //              (beta^0 ad_0 f + beta^1 ad_1 f) + beta^2 ad_2 f, with ad the
 //              upwinded stencil (upwind mode) or the centred one
 constexpr int SLOT_D1 = 0;
 constexpr int SLOT_D2 = 72;
-constexpr int SLOT_ADV = 138;
-constexpr int NSLOT = 162;
+constexpr int SLOT_LAP = 102;
+constexpr int SLOT_ADV = 108;
+constexpr int NSLOT = 132;
 
 __host__ __device__ constexpr int slot_d1(int f, int d) { return SLOT_D1 + f * 3 + d; }
 __host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + d2q(f) * 6 + s; }
+__host__ __device__ constexpr int slot_lap(int f) { return SLOT_LAP + (f - 12); }
 __host__ __device__ constexpr int slot_adv(int f) { return SLOT_ADV + f; }
+
+// Inverse conformal metric (adjugate / determinant) from gt0..5 (xx xy xz yy
+// yz zz), the same expressions as the pointwise algebra's igt; and the
+// contraction sum_s w_s igt_s h_s of a symmetric second-derivative set h with
+// w = (1,2,2,1,2,1), summed in slot order.
+__host__ __device__ inline void inverse_metric(const double g[6], double ig[6])
+{
+#pragma clang fp contract(off)
+    const double c0 = g[3] * g[5] - g[4] * g[4];
+    const double c1 = g[2] * g[4] - g[1] * g[5];
+    const double c2 = g[1] * g[4] - g[2] * g[3];
+    const double c3 = g[0] * g[5] - g[2] * g[2];
+    const double c4 = g[1] * g[2] - g[0] * g[4];
+    const double c5 = g[0] * g[3] - g[1] * g[1];
+    const double idet = 1.0 / (g[0] * c0 + g[1] * c1 + g[2] * c2);
+    ig[0] = c0 * idet;
+    ig[1] = c1 * idet;
+    ig[2] = c2 * idet;
+    ig[3] = c3 * idet;
+    ig[4] = c4 * idet;
+    ig[5] = c5 * idet;
+}
+__host__ __device__ inline double metric_contract(const double ig[6], const double h[6])
+{
+#pragma clang fp contract(off)
+    return ((((ig[0] * h[0] + 2.0 * (ig[1] * h[1])) + 2.0 * (ig[2] * h[2])) + ig[3] * h[3]) + 2.0 * (ig[4] * h[4])) +
+           ig[5] * h[5];
+}
 
 // Sommerfeld falloff / asymptotic value per field (bssn/src/rhs.cpp:261-320).
 __host__ __device__ constexpr double bc_falloff(int f)

@@ -191,13 +191,32 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
 #define BSSN_ADV(F_)                                                                                           \
     ((R)fused_adv(&cur[c], i, j, k, n0, n1, n2, lo0, hi0, lo1, hi1, lo2, hi2, a0, a1, a2, bv0, bv1, bv2, UPWIND))
 
-#define BSSN_D2(F_, S_)                                                                                        \
-    ((R)((S_) == 0   ? d2(&cur[c], 1, i, n0, lo0, hi0, a0)                                                    \
-         : (S_) == 3 ? d2(&cur[c], HX, j, n1, lo1, hi1, a1)                                                   \
-         : (S_) == 5 ? d2(&cur[c], HX * HY, k, n2, lo2, hi2, a2)                                              \
-         : (S_) == 1 ? d1(&DX[lx + TX * ((ly + GW) + HY * (lz + GW))], TX, j, n1, lo1, hi1, a1)               \
-         : (S_) == 2 ? d1(&DX[lx + TX * ((ly + GW) + HY * (lz + GW))], TX * HY, k, n2, lo2, hi2, a2)          \
-                     : d1(&DY[lx + TX * (ly + TY * (lz + GW))], TX * TY, k, n2, lo2, hi2, a2)))
+#define FUSED_D2(S_)                                                                                           \
+    ((S_) == 0   ? d2(&cur[c], 1, i, n0, lo0, hi0, a0)                                                           \
+     : (S_) == 3 ? d2(&cur[c], HX, j, n1, lo1, hi1, a1)                                                          \
+     : (S_) == 5 ? d2(&cur[c], HX * HY, k, n2, lo2, hi2, a2)                                                     \
+     : (S_) == 1 ? d1(&DX[lx + TX * ((ly + GW) + HY * (lz + GW))], TX, j, n1, lo1, hi1, a1)                      \
+     : (S_) == 2 ? d1(&DX[lx + TX * ((ly + GW) + HY * (lz + GW))], TX * HY, k, n2, lo2, hi2, a2)                 \
+                 : d1(&DY[lx + TX * (ly + TY * (lz + GW))], TX * TY, k, n2, lo2, hi2, a2))
+
+#define BSSN_D2(F_, S_) ((R)FUSED_D2(S_))
+
+    // igt^{lm} d_l d_m of the current (conformal-metric) field, igt from the
+    // point's gt values
+    auto fused_lap = [&]() -> double {
+        double g_[6], ig_[6], h_[6];
+#pragma unroll
+        for (int q_ = 0; q_ < 6; q_++) g_[q_] = valid ? in.p[12 + q_][pp] : (q_ == 0 || q_ == 3 || q_ == 5 ? 1.0 : 0.0);
+        h_[0] = FUSED_D2(0);
+        h_[1] = FUSED_D2(1);
+        h_[2] = FUSED_D2(2);
+        h_[3] = FUSED_D2(3);
+        h_[4] = FUSED_D2(4);
+        h_[5] = FUSED_D2(5);
+        inverse_metric(g_, ig_);
+        return metric_contract(ig_, h_);
+    };
+#define BSSN_LAP(F_) ((R)fused_lap())
 
     // Kreiss-Oliger sum (KO_x + KO_y) + KO_z and, on a flagged face, the
     // Sommerfeld value; both in double, from the field's LDS tile.
@@ -228,6 +247,8 @@ __global__ __launch_bounds__(256) void k_fused(FieldPtrs in, OutPtrs out, const 
 #undef BSSN_D1
 #undef BSSN_ADV
 #undef BSSN_D2
+#undef FUSED_D2
+#undef BSSN_LAP
 #undef BSSN_FIELD_DONE
 #undef BSSN_OUT
 }

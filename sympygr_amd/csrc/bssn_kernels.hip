@@ -1,6 +1,6 @@
 // BSSN RHS kernels for gfx950 (CDNA4).
 //
-// Two passes over 8x8x8 interior tiles of every block (tile-major workspace):
+// Two passes over 8x8x8 interior tiles of every block (slot-major workspace):
 //
 //   k_derivs     one workgroup per tile.  Field by field, the 14^3 halo tile
 //                is staged in LDS; every thread evaluates the first, second,
@@ -11,11 +11,14 @@
 //                the same compositions as the reference's sweeps
 //                (bssn/scripts/bssnrhs_derivs.h:7-9).  Results go to the
 //                derivative workspace (double, or float in mixed mode) and
-//                the KO sums (always double).
+//                the KO sums (always double).  Two contractions are done
+//                here, where all their factors are at hand, so the workspace
+//                holds one value instead of three or six: the advection
+//                beta^d ad_d f of every field and igt^{lm} d_l d_m gt_ij.
 //
-//   k_pointwise_g{1,2,3}
+//   k_pointwise_g{1,2}
 //                one thread per interior point, one kernel per output group
-//                (alpha/beta/chi/gt; K/Gt/B; At): reads its fields and
+//                (alpha/beta/chi/gt/K/Gt/B; At): reads its fields and
 //                staged derivatives, runs that group's part of the generated
 //                algebra in R (double or float), applies the Sommerfeld
 //                condition on flagged faces, adds sigma * KO in double and
@@ -65,7 +68,7 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
 __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && f <= 11) || f >= 18; }
 
 template <typename W, bool UPWIND>
-__global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
+__global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                 double *__restrict__ wsK, uint64_t SS, bool all_d1, uint32_t tile0)
 {
@@ -169,13 +172,16 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
         }
 
         if (d2q(f) >= 0) {  // uniform over the workgroup
+            const bool metric = f >= 12;  // gt_ij: only igt^{lm} d_l d_m gt_ij is stored
+            if (!metric) {
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                if (!valid[p]) continue;
-                const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                wsb[slot_d2(f, 0) * SS + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
-                wsb[slot_d2(f, 3) * SS + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
-                wsb[slot_d2(f, 5) * SS + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                for (int p = 0; p < 2; p++) {
+                    if (!valid[p]) continue;
+                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                    wsb[slot_d2(f, 0) * SS + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
+                    wsb[slot_d2(f, 3) * SS + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
+                    wsb[slot_d2(f, 5) * SS + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                }
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
             for (int e = t; e < T * TH * TH; e += 256) {
@@ -195,14 +201,39 @@ __global__ __launch_bounds__(256) void k_derivs(FieldPtrs in, const BlockDesc *_
                 DY[e] = v;
             }
             __syncthreads();
+            if (!metric) {
 #pragma unroll
-            for (int p = 0; p < 2; p++) {
-                if (!valid[p]) continue;
-                const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
-                wsb[slot_d2(f, 1) * SS + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
-                wsb[slot_d2(f, 2) * SS + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
-                wsb[slot_d2(f, 4) * SS + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
+                for (int p = 0; p < 2; p++) {
+                    if (!valid[p]) continue;
+                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                    const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
+                    wsb[slot_d2(f, 1) * SS + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
+                    wsb[slot_d2(f, 2) * SS + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
+                    wsb[slot_d2(f, 4) * SS + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
+                }
+            } else {
+#pragma unroll 1
+                for (int p = 0; p < 2; p++) {
+                    if (!valid[p]) continue;
+                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                    const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
+                    const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
+                    double g[6], ig[6];
+#pragma unroll
+                    for (int q = 0; q < 6; q++) g[q] = in.p[12 + q][gp];
+                    inverse_metric(g, ig);
+                    {  // metric_contract(ig, h) with each h consumed as it is made
+                        SGR_EXACT
+                        double L = ig[0] * d2(c, 1, pi, n0, lo0, hi0, a0);
+                        L = L + 2.0 * (ig[1] * d1(cx, T, pj, n1, lo1, hi1, a1));
+                        L = L + 2.0 * (ig[2] * d1(cx, T * TH, pk[p], n2, lo2, hi2, a2));
+                        L = L + ig[3] * d2(c, TH, pj, n1, lo1, hi1, a1);
+                        L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, lo2, hi2, a2));
+                        L = L + ig[5] * d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                        wsb[slot_lap(f) * SS + wp[p]] = (W)L;
+                    }
+                }
             }
         }
         __syncthreads();  // every read of U (and DX, DY) for field f is done
@@ -266,11 +297,6 @@ __device__ __forceinline__ void rk_store(const RkArgs &rk, int f, uint64_t pp, d
 #include "pointwise_body.h"
 #undef SGR_PW_NAME
 #undef SGR_PW_ALG
-#define SGR_PW_NAME k_pointwise_g3
-#define SGR_PW_ALG "gen/bssn_algebra_g3.inc"
-#include "pointwise_body.h"
-#undef SGR_PW_NAME
-#undef SGR_PW_ALG
 
 // ------------------------------------------------------------------ launch
 template <typename R, typename W, bool UPWIND, bool RK>
@@ -286,9 +312,7 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     const dim3 g(2 * a.ntiles), b(256);
     hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
-    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND, RK>), g, b, 0, s, nullptr, nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
-    hipExtLaunchKernelGGL((k_pointwise_g3<R, W, UPWIND, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
+    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
     return hipGetLastError();
 }
@@ -327,6 +351,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
 #define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
+#define BSSN_LAP(F_) ((R)wsb[slot_lap(F_) * SS])
 #define BSSN_OUT(C_, V_) cout.p[C_][pp] = (V_)
 #define BSSN_BARRIER()
 #include "gen/bssn_constraints.inc"
@@ -334,6 +359,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
 #undef BSSN_D1
 #undef BSSN_D2
 #undef BSSN_OUT
+#undef BSSN_LAP
 #undef BSSN_BARRIER
 }
 

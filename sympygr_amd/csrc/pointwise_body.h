@@ -48,6 +48,7 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
 #define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
 #define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
 #define BSSN_ADV(F_) ((R)wsb[slot_adv(F_) * SS])
+#define BSSN_LAP(F_) ((R)wsb[slot_lap(F_) * SS])
 #define BSSN_OUT(F_, V_)                                                                                      \
     do {                                                                                                    \
         double val_ = (double)(V_);                                                                         \
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
 #undef BSSN_D1
 #undef BSSN_D2
 #undef BSSN_ADV
+#undef BSSN_LAP
 #undef BSSN_OUT
 }
 
