@@ -137,6 +137,7 @@ enum class Variant { Fused, Workspace };
 
 struct sgr_plan {
     Variant variant = Variant::Workspace;
+    int derivs_threads = 512;  // k_derivs workgroup size (512: one point per thread, 256: two)
     int device = 0;
     sgr_params params{};
     std::vector<BlockDesc> hblocks;
@@ -351,12 +352,16 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     p->device = device;
     p->params = *params;
     // Default: the two-pass kernels (LDS-tiled derivative kernel into an HBM
-    // workspace, then three output-group pointwise kernels).  SGR_KERNEL=fused
+    // workspace, then two output-group pointwise kernels).  SGR_KERNEL=fused
     // selects the single-pass kernel (no workspace; slower on gfx950 today
     // because its register footprint allows one wave per SIMD).
+    // SGR_DERIVS_THREADS=256|512 sets the derivative kernel's workgroup size.
     {
         const char *kv = getenv("SGR_KERNEL");
         if (kv && !strcmp(kv, "fused")) p->variant = Variant::Fused;
+        const char *tv = getenv("SGR_DERIVS_THREADS");
+        if (tv && !strcmp(tv, "512")) p->derivs_threads = 512;
+        else if (tv && !strcmp(tv, "256")) p->derivs_threads = 256;
     }
     const bool fusedv = p->variant == Variant::Fused;
     const int tdim[3] = {fusedv ? FUSED_TX : T, fusedv ? FUSED_TY : T, fusedv ? FUSED_TZ : T};
@@ -495,6 +500,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.params = device_params(plan->params);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    a.derivs_threads = plan->derivs_threads;
     for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
     a.rk.stage = -1;
     if (rk) {
@@ -567,6 +573,7 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     a.np = plan->np;
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    a.derivs_threads = plan->derivs_threads;
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");

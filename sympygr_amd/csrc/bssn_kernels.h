@@ -33,6 +33,7 @@ struct LaunchArgs {
     bool mixed;
     hipEvent_t ev[4];            // optional: start/stop of k_derivs, start/stop of k_pointwise
     RkArgs rk;                   // rk.stage < 0: plain RHS
+    int derivs_threads;          // k_derivs workgroup size: 256 or 512
 };
 
 // Ghost refresh: one box of ghost nodes of a target block whose positions lie

@@ -4,8 +4,8 @@
 //
 //   k_derivs     one workgroup per tile.  Field by field, the 14^3 halo tile
 //                is staged in LDS; every thread evaluates the first, second,
-//                advective and Kreiss-Oliger stencils of its two points from
-//                LDS.  The inner first derivatives of the mixed terms
+//                advective and Kreiss-Oliger stencils of its point (512
+//                threads; or of two points, 256 threads) from LDS.  The inner first derivatives of the mixed terms
 //                (D_x u on the tile's y/z halo, D_y u on its z halo) are
 //                staged in LDS as well, so D_y D_x, D_z D_x and D_z D_y are
 //                the same compositions as the reference's sweeps
@@ -67,13 +67,16 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
 // Sommerfeld condition, i.e. on tiles that touch a flagged face.
 __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && f <= 11) || f >= 18; }
 
-template <typename W, bool UPWIND>
-__global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
+// NT threads per tile: 256 (two points per thread) or 512 (one point each).
+template <typename W, bool UPWIND, int NT>
+__global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                 double *__restrict__ wsK, uint64_t SS, bool all_d1, uint32_t tile0)
 {
     constexpr int NHALO = TH * TH * TH;            // 2744
-    constexpr int NPF = (NHALO + 255) / 256;       // 11 prefetch registers per thread
+    constexpr int NPF = (NHALO + NT - 1) / NT;     // prefetch registers per thread
+    constexpr int PPT = TILE_PTS / NT;             // points per thread
+    constexpr int ZS = NT / 64;                    // z-planes per pass over the tile
     __shared__ double U[NHALO];          // x + 14 (y + 14 z), halo coordinates
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
@@ -91,21 +94,21 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
                            (hi1 && tc.j0 + T >= n1 - GW) || (lo2 && tc.k0 == GW) || (hi2 && tc.k0 + T >= n2 - GW);
 
     const int t = threadIdx.x;
-    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;  // lz in [0,4); points lz and lz+4
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;  // lz in [0,ZS); points lz + ZS p
     const int pi = tc.i0 + lx, pj = tc.j0 + ly;
-    bool valid[2];
-    int pk[2];
-    uint64_t wp[2];
-    double bv[2][3];  // shift at the point
+    bool valid[PPT];
+    int pk[PPT];
+    uint64_t wp[PPT];
+    double bv[PPT][3];  // shift at the point
     // slot-major workspace with a padded plane stride SS (points); this tile's
     // 512 points are contiguous within every plane
     W *__restrict__ wsb = ws + (uint64_t)ct * TILE_PTS;
     double *__restrict__ kb = wsK + (uint64_t)ct * TILE_PTS;
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
-        pk[p] = tc.k0 + lz + 4 * p;
+    for (int p = 0; p < PPT; p++) {
+        pk[p] = tc.k0 + lz + ZS * p;
         valid[p] = pi < n0 - GW && pj < n1 - GW && pk[p] < n2 - GW;
-        wp[p] = (uint64_t)(lx + T * (ly + T * (lz + 4 * p)));
+        wp[p] = (uint64_t)(lx + T * (ly + T * (lz + ZS * p)));
         const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
 #pragma unroll
         for (int d = 0; d < 3; d++) bv[p][d] = valid[p] ? in.p[6 + d][gp] : 0.0;
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
     int32_t src[NPF];
 #pragma unroll
     for (int q = 0; q < NPF; q++) {
-        const int e = t + 256 * q;
+        const int e = t + NT * q;
         const int ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
         const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
         src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
         const double *__restrict__ s0 = in.p[0] + B.offset;
 #pragma unroll
         for (int q = 0; q < NPF; q++)
-            if (t + 256 * q < NHALO) U[t + 256 * q] = src[q] >= 0 ? s0[src[q]] : 0.0;
+            if (t + NT * q < NHALO) U[t + NT * q] = src[q] >= 0 ? s0[src[q]] : 0.0;
     }
     __syncthreads();
 
@@ -138,9 +141,9 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
         }
         const bool need_d1 = all_d1 || !d1_only_for_faces(f) || face_tile;
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
+        for (int p = 0; p < PPT; p++) {
             if (!valid[p]) continue;
-            const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
+            const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
             if (need_d1 || (!UPWIND && !all_d1)) {
                 const double g0 = d1(c, 1, pi, n0, lo0, hi0, a0);
                 const double g1 = d1(c, TH, pj, n1, lo1, hi1, a1);
@@ -175,16 +178,16 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
             const bool metric = f >= 12;  // gt_ij: only igt^{lm} d_l d_m gt_ij is stored
             if (!metric) {
 #pragma unroll
-                for (int p = 0; p < 2; p++) {
+                for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
-                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
+                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     wsb[slot_d2(f, 0) * SS + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
                     wsb[slot_d2(f, 3) * SS + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
                     wsb[slot_d2(f, 5) * SS + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
                 }
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
-            for (int e = t; e < T * TH * TH; e += 256) {
+            for (int e = t; e < T * TH * TH; e += NT) {
                 const int x = e % T, y = (e / T) % TH, z = e / (T * TH);
                 const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
                     v = d1(&U[(x + GW) + TH * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
                 DX[e] = v;
             }
-            for (int e = t; e < T * T * TH; e += 256) {
+            for (int e = t; e < T * T * TH; e += NT) {
                 const int x = e % T, y = (e / T) % T, z = e / (T * T);
                 const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
@@ -203,21 +206,21 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
             __syncthreads();
             if (!metric) {
 #pragma unroll
-                for (int p = 0; p < 2; p++) {
+                for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
-                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                    const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
+                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
+                    const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
                     wsb[slot_d2(f, 1) * SS + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
                     wsb[slot_d2(f, 2) * SS + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
                     wsb[slot_d2(f, 4) * SS + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
                 }
             } else {
 #pragma unroll 1
-                for (int p = 0; p < 2; p++) {
+                for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
-                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + 4 * p + GW))];
-                    const double *cy = &DY[lx + T * (ly + T * (lz + 4 * p + GW))];
+                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
+                    const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
+                    const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
                     const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
                     double g[6], ig[6];
 #pragma unroll
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256, 3) void k_derivs(FieldPtrs in, const BlockDesc
         if (f + 1 < NF) {
 #pragma unroll
             for (int q = 0; q < NPF; q++)
-                if (t + 256 * q < NHALO) U[t + 256 * q] = pf[q];
+                if (t + NT * q < NHALO) U[t + NT * q] = pf[q];
         }
         __syncthreads();
     }
@@ -305,8 +308,12 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     W *ws = (W *)a.ws;
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
-    hipExtLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                          a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
+    if (a.derivs_threads == 512)
+        hipExtLaunchKernelGGL((k_derivs<W, UPWIND, 512>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
+                              a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
+    else
+        hipExtLaunchKernelGGL((k_derivs<W, UPWIND, 256>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
+                              a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const dim3 g(2 * a.ntiles), b(256);
@@ -367,8 +374,12 @@ template <typename W, bool UPWIND>
 static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
 {
     W *ws = (W *)a.ws;
-    hipLaunchKernelGGL((k_derivs<W, UPWIND>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws, a.wsK,
-                       a.np, true, a.tile0);
+    if (a.derivs_threads == 512)
+        hipLaunchKernelGGL((k_derivs<W, UPWIND, 512>), dim3(a.ntiles), dim3(512), 0, s, a.in, a.blocks, a.tile_block, ws,
+                           a.wsK, a.np, true, a.tile0);
+    else
+        hipLaunchKernelGGL((k_derivs<W, UPWIND, 256>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws,
+                           a.wsK, a.np, true, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_constraints<W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
