@@ -44,6 +44,31 @@ __host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + d2q(f
 __host__ __device__ constexpr int slot_lap(int f) { return SLOT_LAP + (f - 12); }
 __host__ __device__ constexpr int slot_adv(int f) { return SLOT_ADV + f; }
 
+// The derivative workspace is streamed: written once by the derivative pass,
+// read once per pointwise kernel, far larger than L2.  Non-temporal accesses
+// keep it from displacing the field tiles the derivative pass re-reads.
+#ifndef SGR_WS_NONTEMPORAL
+#define SGR_WS_NONTEMPORAL 1
+#endif
+template <typename X>
+__device__ __forceinline__ void ws_put(X *p, X v)
+{
+#if SGR_WS_NONTEMPORAL
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <typename X>
+__device__ __forceinline__ X ws_get(const X *p)
+{
+#if SGR_WS_NONTEMPORAL
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
 // Inverse conformal metric (adjugate / determinant) from gt0..5 (xx xy xz yy
 // yz zz), the same expressions as the pointwise algebra's igt; and the
 // contraction sum_s w_s igt_s h_s of a symmetric second-derivative set h with

@@ -149,13 +149,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const double g1 = d1(c, TH, pj, n1, lo1, hi1, a1);
                 const double g2 = d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
                 if (need_d1) {
-                    wsb[slot_d1(f, 0) * SS + wp[p]] = (W)g0;
-                    wsb[slot_d1(f, 1) * SS + wp[p]] = (W)g1;
-                    wsb[slot_d1(f, 2) * SS + wp[p]] = (W)g2;
+                    ws_put(&wsb[slot_d1(f, 0) * SS + wp[p]], (W)(g0));
+                    ws_put(&wsb[slot_d1(f, 1) * SS + wp[p]], (W)(g1));
+                    ws_put(&wsb[slot_d1(f, 2) * SS + wp[p]], (W)(g2));
                 }
                 if (!UPWIND && !all_d1) {
                     SGR_EXACT
-                    wsb[slot_adv(f) * SS + wp[p]] = (W)((bv[p][0] * g0 + bv[p][1] * g1) + bv[p][2] * g2);
+                    ws_put(&wsb[slot_adv(f) * SS + wp[p]], (W)(((bv[p][0] * g0 + bv[p][1] * g1) + bv[p][2] * g2)));
                 }
             }
             if (UPWIND && !all_d1) {
@@ -163,14 +163,14 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const double q0 = adv(c, 1, pi, n0, lo0, hi0, bv[p][0] > 0.0 ? 1.0 : 0.0, a0);
                 const double q1 = adv(c, TH, pj, n1, lo1, hi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
                 const double q2 = adv(c, TH * TH, pk[p], n2, lo2, hi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
-                wsb[slot_adv(f) * SS + wp[p]] = (W)((bv[p][0] * q0 + bv[p][1] * q1) + bv[p][2] * q2);
+                ws_put(&wsb[slot_adv(f) * SS + wp[p]], (W)(((bv[p][0] * q0 + bv[p][1] * q1) + bv[p][2] * q2)));
             }
             {
                 SGR_EXACT
                 const double kx = ko(c, 1, pi, n0, lo0, hi0, a0);
                 const double ky = ko(c, TH, pj, n1, lo1, hi1, a1);
                 const double kz = ko(c, TH * TH, pk[p], n2, lo2, hi2, a2);
-                kb[f * SS + wp[p]] = kx + ky + kz;
+                ws_put(&kb[f * SS + wp[p]], kx + ky + kz);
             }
         }
 
@@ -181,9 +181,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
                     const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
-                    wsb[slot_d2(f, 0) * SS + wp[p]] = (W)d2(c, 1, pi, n0, lo0, hi0, a0);
-                    wsb[slot_d2(f, 3) * SS + wp[p]] = (W)d2(c, TH, pj, n1, lo1, hi1, a1);
-                    wsb[slot_d2(f, 5) * SS + wp[p]] = (W)d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                    ws_put(&wsb[slot_d2(f, 0) * SS + wp[p]], (W)(d2(c, 1, pi, n0, lo0, hi0, a0)));
+                    ws_put(&wsb[slot_d2(f, 3) * SS + wp[p]], (W)(d2(c, TH, pj, n1, lo1, hi1, a1)));
+                    ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, TH * TH, pk[p], n2, lo2, hi2, a2)));
                 }
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
@@ -210,9 +210,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     if (!valid[p]) continue;
                     const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
-                    wsb[slot_d2(f, 1) * SS + wp[p]] = (W)d1(cx, T, pj, n1, lo1, hi1, a1);
-                    wsb[slot_d2(f, 2) * SS + wp[p]] = (W)d1(cx, T * TH, pk[p], n2, lo2, hi2, a2);
-                    wsb[slot_d2(f, 4) * SS + wp[p]] = (W)d1(cy, T * T, pk[p], n2, lo2, hi2, a2);
+                    ws_put(&wsb[slot_d2(f, 1) * SS + wp[p]], (W)(d1(cx, T, pj, n1, lo1, hi1, a1)));
+                    ws_put(&wsb[slot_d2(f, 2) * SS + wp[p]], (W)(d1(cx, T * TH, pk[p], n2, lo2, hi2, a2)));
+                    ws_put(&wsb[slot_d2(f, 4) * SS + wp[p]], (W)(d1(cy, T * T, pk[p], n2, lo2, hi2, a2)));
                 }
             } else {
 #pragma unroll 1
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                         L = L + ig[3] * d2(c, TH, pj, n1, lo1, hi1, a1);
                         L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, lo2, hi2, a2));
                         L = L + ig[5] * d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
-                        wsb[slot_lap(f) * SS + wp[p]] = (W)L;
+                        ws_put(&wsb[slot_lap(f) * SS + wp[p]], (W)(L));
                     }
                 }
             }
@@ -356,9 +356,9 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
     const W *__restrict__ wsb = ws + (uint64_t)(wg >> 1) * TILE_PTS + l;
     typedef double R;
 #define BSSN_U(F_) ((R)in.p[F_][pp])
-#define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
-#define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
-#define BSSN_LAP(F_) ((R)wsb[slot_lap(F_) * SS])
+#define BSSN_D1(F_, D_) ((R)ws_get(&wsb[slot_d1(F_, D_) * SS]))
+#define BSSN_D2(F_, S_) ((R)ws_get(&wsb[slot_d2(F_, S_) * SS]))
+#define BSSN_LAP(F_) ((R)ws_get(&wsb[slot_lap(F_) * SS]))
 #define BSSN_OUT(C_, V_) cout.p[C_][pp] = (V_)
 #define BSSN_BARRIER()
 #include "gen/bssn_constraints.inc"

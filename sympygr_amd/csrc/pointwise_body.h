@@ -45,18 +45,19 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
     const double sigma = P.ko_sigma;
 
 #define BSSN_U(F_) ((R)in.p[F_][pp])
-#define BSSN_D1(F_, D_) ((R)wsb[slot_d1(F_, D_) * SS])
-#define BSSN_D2(F_, S_) ((R)wsb[slot_d2(F_, S_) * SS])
-#define BSSN_ADV(F_) ((R)wsb[slot_adv(F_) * SS])
-#define BSSN_LAP(F_) ((R)wsb[slot_lap(F_) * SS])
+#define BSSN_D1(F_, D_) ((R)ws_get(&wsb[slot_d1(F_, D_) * SS]))
+#define BSSN_D2(F_, S_) ((R)ws_get(&wsb[slot_d2(F_, S_) * SS]))
+#define BSSN_ADV(F_) ((R)ws_get(&wsb[slot_adv(F_) * SS]))
+#define BSSN_LAP(F_) ((R)ws_get(&wsb[slot_lap(F_) * SS]))
 #define BSSN_OUT(F_, V_)                                                                                      \
     do {                                                                                                    \
         double val_ = (double)(V_);                                                                         \
         if (face)                                                                                           \
-            val_ = sommerfeld(x, y, z, inv_r, (double)wsb[slot_d1(F_, 0) * SS],                             \
-                              (double)wsb[slot_d1(F_, 1) * SS], (double)wsb[slot_d1(F_, 2) * SS],           \
+            val_ = sommerfeld(x, y, z, inv_r, (double)ws_get(&wsb[slot_d1(F_, 0) * SS]),                    \
+                              (double)ws_get(&wsb[slot_d1(F_, 1) * SS]),                                    \
+                              (double)ws_get(&wsb[slot_d1(F_, 2) * SS]),                                    \
                               in.p[F_][pp], bc_falloff(F_), bc_asym(F_));                                   \
-        val_ = ko_add(val_, sigma, kb[(F_) * SS]);                                                          \
+        val_ = ko_add(val_, sigma, ws_get(&kb[(F_) * SS]));                                                          \
         if (RK)                                                                                             \
             rk_store(rk, F_, pp, val_);                                                                     \
         else                                                                                                \
