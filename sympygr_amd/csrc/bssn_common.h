@@ -69,6 +69,19 @@ __device__ __forceinline__ X ws_get(const X *p)
 #endif
 }
 
+// RHS and RK-stage outputs: plain stores, or non-temporal ones.
+#ifndef SGR_OUT_NONTEMPORAL
+#define SGR_OUT_NONTEMPORAL 0
+#endif
+__device__ __forceinline__ void out_put(double *p, double v)
+{
+#if SGR_OUT_NONTEMPORAL
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // Inverse conformal metric (adjugate / determinant) from gt0..5 (xx xy xz yy
 // yz zz), the same expressions as the pointwise algebra's igt; and the
 // contraction sum_s w_s igt_s h_s of a symmetric second-derivative set h with

@@ -282,12 +282,12 @@ __device__ __forceinline__ void rk_store(const RkArgs &rk, int f, uint64_t pp, d
     SGR_EXACT
     const double wk = rk.w * k;
     if (rk.stage == 3) {
-        rk.next.p[f][pp] = rk.acc.p[f][pp] + wk;
+        out_put(&rk.next.p[f][pp], rk.acc.p[f][pp] + wk);
         return;
     }
     const double u0 = rk.u0.p[f][pp];
-    rk.acc.p[f][pp] = (rk.stage == 0 ? u0 : rk.acc.p[f][pp]) + wk;
-    rk.next.p[f][pp] = u0 + rk.cn * k;
+    out_put(&rk.acc.p[f][pp], (rk.stage == 0 ? u0 : rk.acc.p[f][pp]) + wk);
+    out_put(&rk.next.p[f][pp], u0 + rk.cn * k);
 }
 
 #define SGR_PW_NAME k_pointwise_g1

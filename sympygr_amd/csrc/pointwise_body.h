@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
         if (RK)                                                                                             \
             rk_store(rk, F_, pp, val_);                                                                     \
         else                                                                                                \
-            out.p[F_][pp] = val_;                                                                           \
+            out_put(&out.p[F_][pp], val_);                                                                  \
     } while (0)
 #include SGR_PW_ALG
 #undef BSSN_BARRIER
