@@ -18,6 +18,12 @@
 void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
              const double *ptmax, const unsigned int *sz, const unsigned int &bflag);
 
+/* The staged form, bssn/include/rhs.h:39-42 (bssn/src/rhs.cpp:392-825): the
+ * same right-hand side computed one variable at a time; here the same
+ * kernels serve it. */
+void bssnrhs_sep(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
+                 const double *ptmax, const unsigned int *sz, const unsigned int &bflag);
+
 extern "C" {
 /* Method switches the reference selects with compile-time macros
  * (BSSN_USE_ADVECTIVE_DERIVS / deriv42adv_*, bssn/include/derivs.h:68-103):
@@ -25,6 +31,14 @@ extern "C" {
  * SGR_PREC_MIXED, and the HIP device used by bssnrhs.  Default: centred
  * (the shipped 6th-order build), FP64, device 0. */
 void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device);
+
+/* Gauge switches the reference selects with compile-time macros
+ * (bssn/src/rhs.cpp:224-243): gauge SGR_GAUGE_STANDARD or SGR_GAUGE_ROCHESTER
+ * (USE_ROCHESTER_GAUGE), eta_mode SGR_ETA_CONST or SGR_ETA_FUNC
+ * (USE_ETA_FUNC).  Their parameters come from the bssn:: globals BSSN_XI,
+ * BSSN_ETA_R0, BSSN_ETA_POWER (bssn/src/parameters.cpp:50-51,123).
+ * Default: standard gauge, constant eta. */
+void sgr_dropin_configure_gauge(int32_t gauge, int32_t eta_mode);
 }
 
 #endif

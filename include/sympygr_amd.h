@@ -37,6 +37,8 @@ enum {
 
 enum { SGR_ADV_CENTRED = 0, SGR_ADV_UPWIND = 1 };
 enum { SGR_PREC_FP64 = 0, SGR_PREC_MIXED = 1 };
+enum { SGR_GAUGE_STANDARD = 0, SGR_GAUGE_ROCHESTER = 1 };
+enum { SGR_ETA_CONST = 0, SGR_ETA_FUNC = 1 };
 
 /* Physics and method parameters.  The reference reads these from mutable
  * bssn:: globals at call time (bssn/src/parameters.cpp:99-109,
@@ -52,9 +54,21 @@ typedef struct {
                                  or SGR_ADV_UPWIND (deriv42adv_*, bssn/src/derivs.cpp:495-906) */
     int32_t precision;        /* SGR_PREC_FP64, or SGR_PREC_MIXED: FP64 state, stencils,
                                  KO and boundary terms; FP32 pointwise algebra */
+    /* Gauge variants the reference selects at compile time (bssn/src/rhs.cpp:224-243): */
+    int32_t gauge;            /* SGR_GAUGE_STANDARD (bssneqs_eta_*_standard_gauge.cpp) or
+                                 SGR_GAUGE_ROCHESTER (USE_ROCHESTER_GAUGE: GR/bssn.py:194-209,
+                                 b_rhs = BSSN_XI[1] beta.ad beta + 3/4 BSSN_XI[2] Gt - eta beta,
+                                 B not evolved) */
+    int32_t eta_mode;         /* SGR_ETA_CONST (ETA_CONST, damped beyond ETA_R0) or SGR_ETA_FUNC
+                                 (USE_ETA_FUNC: GR/bssn.py:52, eta = BSSN_ETA_R0
+                                 sqrt(igt^ij d_i chi d_j chi) / (1 - chi^p0)^p1) */
+    uint32_t xi[3];           /* BSSN_XI (unsigned in the reference) */
+    double eta_func_r0;       /* BSSN_ETA_R0 */
+    double eta_power[2];      /* BSSN_ETA_POWER: p0, p1 */
 } sgr_params;
 
-/* Defaults of bssn/src/parameters.cpp:99-109 with adv_mode/precision given. */
+/* Defaults of bssn/src/parameters.cpp:50-51,99-109,123 with adv_mode/precision
+ * given: standard gauge, constant eta. */
 void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision);
 
 /* One ghost-padded block: the per-block arguments of the reference

@@ -276,6 +276,21 @@ double metric_laplacian(const double *const *in, double *const *grad2, std::ptrd
               ig[3] * grad2[3][pp]) + 2.0 * (ig[4] * grad2[4][pp])) + ig[5] * grad2[5][pp];
 }
 
+// igt^ij c_i c_j at one point, igt by adjugate / determinant of gt0..5.
+double inverse_metric_quadratic(const double *const *in, std::ptrdiff_t pp, double c0, double c1, double c2)
+{
+    const double g0 = in[12][pp], g1 = in[13][pp], g2 = in[14][pp], g3 = in[15][pp], g4 = in[16][pp], g5 = in[17][pp];
+    const double a0 = g3 * g5 - g4 * g4;
+    const double a1 = g2 * g4 - g1 * g5;
+    const double a2 = g1 * g4 - g2 * g3;
+    const double a3 = g0 * g5 - g2 * g2;
+    const double a4 = g1 * g2 - g0 * g4;
+    const double a5 = g0 * g3 - g1 * g1;
+    const double idet = 1.0 / (g0 * a0 + g1 * a1 + g2 * a2);
+    return idet * (a0 * c0 * c0 + 2.0 * a1 * c0 * c1 + 2.0 * a2 * c0 * c2 + a3 * c1 * c1 + 2.0 * a4 * c1 * c2 +
+                   a5 * c2 * c2);
+}
+
 int rhs_block(double **outv, const double **inv, uint64_t offset, const double *pmin, const double *pmax,
               const uint32_t *sz, uint32_t bflag, const oracle_params *P, Workspace &W)
 {
@@ -307,13 +322,26 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
                 const std::ptrdiff_t pp = g.idx(i, j, k);
                 const double r_coord = std::sqrt(x * x + y * y + z * z);
                 double eta = P->eta_const;
-                if (r_coord >= P->eta_r0) eta *= std::pow(P->eta_r0 / r_coord, P->eta_damping_exp);
+                if (P->eta_func) {
+                    // GR/bssn.py:52 (USE_ETA_FUNC): R0 sqrt(igt^ij d_i chi d_j chi) / (1 - chi^p0)^p1
+                    const double c0 = W.grad[1][0][pp], c1 = W.grad[1][1][pp], c2 = W.grad[1][2][pp];
+                    eta = P->eta_func_r0 * std::sqrt(inverse_metric_quadratic(in, pp, c0, c1, c2)) *
+                          std::pow(1.0 - std::pow(in[1][pp], P->eta_power[0]), -P->eta_power[1]);
+                } else if (r_coord >= P->eta_r0) {
+                    eta *= std::pow(P->eta_r0 / r_coord, P->eta_damping_exp);
+                }
+                const double xi1 = P->xi[1], xi2 = P->xi[2];
+                const bool rochester = P->rochester != 0;
 #define BSSN_U(f) in[f][pp]
 #define BSSN_D1(f, d) W.grad[f][d][pp]
 #define BSSN_D2(f, s) W.grad2[f][s][pp]
 #define BSSN_ADV(f) ((in[6][pp] * W.agrad[f][0][pp] + in[7][pp] * W.agrad[f][1][pp]) + in[8][pp] * W.agrad[f][2][pp])
 #define BSSN_LAP(f) metric_laplacian(in, W.grad2[f], pp)
-#define BSSN_OUT(f, v) out[f][pp] = (v)
+// the Rochester gauge replaces the shift (6..8) and B (9..11) equations
+#define BSSN_OUT(f, v) \
+    if (!(rochester && (f) >= 6 && (f) <= 11)) out[f][pp] = (v)
+#define BSSN_ALT_ON rochester
+#define BSSN_OUT_ALT(f, v) out[f][pp] = (v)
 #define BSSN_BARRIER()
 #include "gen/bssn_algebra.inc"
 #undef BSSN_U
@@ -322,6 +350,8 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #undef BSSN_ADV
 #undef BSSN_LAP
 #undef BSSN_OUT
+#undef BSSN_OUT_ALT
+#undef BSSN_ALT_ON
 #undef BSSN_BARRIER
             }
         }

@@ -25,6 +25,11 @@ typedef struct {
     double ko_sigma;         /* KO_DISS_SIGMA */
     int32_t upwind;          /* 0: advective derivs = centred deriv64 (as shipped)
                                 1: 4th-order shift-upwinded deriv42adv */
+    int32_t rochester;       /* USE_ROCHESTER_GAUGE (rhs.cpp:224-243, GR/bssn.py:194-209) */
+    int32_t eta_func;        /* USE_ETA_FUNC (GR/bssn.py:52) */
+    uint32_t xi[3];          /* BSSN_XI */
+    double eta_func_r0;      /* BSSN_ETA_R0 */
+    double eta_power[2];     /* BSSN_ETA_POWER */
 } oracle_params;
 
 /* One block, same argument meaning as the reference bssnrhs (rhs.h:33-36). */

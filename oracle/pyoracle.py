@@ -23,12 +23,24 @@ class OracleParams(ctypes.Structure):
         ("eta_damping_exp", ctypes.c_double),
         ("ko_sigma", ctypes.c_double),
         ("upwind", ctypes.c_int32),
+        ("rochester", ctypes.c_int32),
+        ("eta_func", ctypes.c_int32),
+        ("xi", ctypes.c_uint32 * 3),
+        ("eta_func_r0", ctypes.c_double),
+        ("eta_power", ctypes.c_double * 2),
     ]
 
 
 def make_params(upwind=True, ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
-                lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0)):
+                lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
+                eta_func_r0=1.31, eta_power=(2.0, 2.0)):
     p = OracleParams()
+    p.rochester = 1 if gauge == "rochester" else 0
+    p.eta_func = 1 if eta == "func" else 0
+    for i in range(3):
+        p.xi[i] = xi[i]
+    p.eta_func_r0 = eta_func_r0
+    p.eta_power[0], p.eta_power[1] = eta_power
     for i in range(4):
         p.lambda_[i] = lambda_[i]
     p.lambda_f[0], p.lambda_f[1] = lambda_f
@@ -139,19 +151,27 @@ def ref_lib(variant):
         L.ref_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint), P_D,
                                 ctypes.c_uint]
         L.ref_fake_initial_data.argtypes = [ctypes.c_double] * 3 + [P_D]
+        L.ref_bssnrhs_sep.argtypes = L.ref_bssnrhs.argtypes
+        L.ref_set_gauge_params.argtypes = [ctypes.c_double, P_D, ctypes.POINTER(ctypes.c_uint)]
         _ref[variant] = L
     return _ref[variant]
 
 
-def ref_rhs_block(variant, u, sz, pmin, pmax, bflag, params):
+def ref_rhs_block(variant, u, sz, pmin, pmax, bflag, params, staged=False):
+    """The compiled reference on one block (bssnrhs, or bssnrhs_sep when
+    staged).  The gauge switches are the build variant's; the gauge
+    parameters (BSSN_ETA_R0, BSSN_ETA_POWER, BSSN_XI) come from params."""
     L = ref_lib(variant)
     u = np.ascontiguousarray(u, dtype=np.float64)
     out = np.zeros_like(u)
     lam = (ctypes.c_uint * 4)(*params.lambda_)
     lf = (ctypes.c_double * 2)(*params.lambda_f)
-    L.ref_bssnrhs(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
-                  (ctypes.c_uint * 3)(*sz), bflag, lam, lf, params.eta_const, params.eta_r0,
-                  params.eta_damping_exp, params.ko_sigma)
+    L.ref_set_gauge_params(params.eta_func_r0, (ctypes.c_double * 2)(*params.eta_power),
+                           (ctypes.c_uint * 3)(*params.xi))
+    fn = L.ref_bssnrhs_sep if staged else L.ref_bssnrhs
+    fn(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+       (ctypes.c_uint * 3)(*sz), bflag, lam, lf, params.eta_const, params.eta_r0,
+       params.eta_damping_exp, params.ko_sigma)
     return out
 
 

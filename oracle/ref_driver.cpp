@@ -29,6 +29,39 @@ int ref_bssnrhs(double **out, const double **in, unsigned int offset,
     return 0;
 }
 
+// The staged form of the same right-hand side (bssn/src/rhs.cpp:392-825):
+// one generated file per variable (a_rhs.cpp, b_rhs.cpp, ...), same
+// parameters, same boundary and dissipation steps.
+int ref_bssnrhs_sep(double **out, const double **in, unsigned int offset,
+                    const double *pmin, const double *pmax, const unsigned int *sz,
+                    unsigned int bflag, const unsigned int *lambda,
+                    const double *lambda_f, double eta_const, double eta_r0,
+                    double eta_damping_exp, double ko_sigma)
+{
+    for (int i = 0; i < 4; i++) bssn::BSSN_LAMBDA[i] = lambda[i];
+    bssn::BSSN_LAMBDA_F[0] = lambda_f[0];
+    bssn::BSSN_LAMBDA_F[1] = lambda_f[1];
+    bssn::ETA_CONST = eta_const;
+    bssn::ETA_R0 = eta_r0;
+    bssn::ETA_DAMPING_EXP = eta_damping_exp;
+    bssn::KO_DISS_SIGMA = ko_sigma;
+    const unsigned int off = offset;
+    const unsigned int bf = bflag;
+    bssnrhs_sep(out, in, off, pmin, pmax, sz, bf);
+    return 0;
+}
+
+// Parameters of the gauge variants (bssn/src/parameters.cpp:50-51,123):
+// the eta function R0 * sqrt(igt^ij d_i chi d_j chi) / (1 - chi^p0)^p1
+// (USE_ETA_FUNC) and the Rochester shift weights BSSN_XI (USE_ROCHESTER_GAUGE).
+void ref_set_gauge_params(double eta_func_r0, const double *eta_power, const unsigned int *xi)
+{
+    bssn::BSSN_ETA_R0 = eta_func_r0;
+    bssn::BSSN_ETA_POWER[0] = eta_power[0];
+    bssn::BSSN_ETA_POWER[1] = eta_power[1];
+    for (int i = 0; i < 3; i++) bssn::BSSN_XI[i] = xi[i];
+}
+
 // Single stencil sweeps (bssn/src/derivs.cpp), selected by a small integer:
 // 0..2 deriv_{x,y,z}, 3..5 deriv_{xx,yy,zz}, 6..8 adv_deriv_{x,y,z},
 // 9..11 ko_deriv_{x,y,z}.  `beta` is only read by the advective sweeps.

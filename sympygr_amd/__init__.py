@@ -42,6 +42,11 @@ class SgrParams(ctypes.Structure):
         ("ko_sigma", ctypes.c_double),
         ("adv_mode", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("gauge", ctypes.c_int32),
+        ("eta_mode", ctypes.c_int32),
+        ("xi", ctypes.c_uint32 * 3),
+        ("eta_func_r0", ctypes.c_double),
+        ("eta_power", ctypes.c_double * 2),
     ]
 
 
@@ -137,14 +142,24 @@ def check(rc):
     return rc
 
 
+GAUGE = {"standard": 0, "rochester": 1}
+ETA = {"const": 0, "func": 1}
+
+
 def params(adv="upwind", precision="fp64", ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
-           lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0)) -> SgrParams:
+           lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
+           eta_func_r0=1.31, eta_power=(2.0, 2.0)) -> SgrParams:
     p = SgrParams()
     lib().sgr_params_default(ctypes.byref(p), ADV[adv], PREC[precision])
     for i in range(4):
         p.lambda_[i] = lambda_[i]
     p.lambda_f[0], p.lambda_f[1] = lambda_f
     p.eta_const, p.eta_r0, p.eta_damping_exp, p.ko_sigma = eta_const, eta_r0, eta_damping_exp, ko_sigma
+    p.gauge, p.eta_mode = GAUGE[gauge], ETA[eta]
+    for i in range(3):
+        p.xi[i] = xi[i]
+    p.eta_func_r0 = eta_func_r0
+    p.eta_power[0], p.eta_power[1] = eta_power
     return p
 
 

@@ -66,6 +66,7 @@ class Inputs:
             "lambda2": sp.Symbol("lambda2"), "lambda3": sp.Symbol("lambda3"),
             "lambda_f0": sp.Symbol("lambda_f0"), "lambda_f1": sp.Symbol("lambda_f1"),
             "eta": sp.Symbol("eta"),
+            "xi1": sp.Symbol("xi1"), "xi2": sp.Symbol("xi2"),
         }
 
     # d_i f  (centred, with bflag closures)
@@ -107,6 +108,9 @@ class Stages:
     def __init__(self):
         self.stages: list[tuple[str, list[tuple[sp.Symbol, sp.Expr]]]] = []
         self.outputs: list[tuple[str, sp.Expr]] = []
+        # alternative value of an output under another gauge: output index ->
+        # expression (stored instead of the output when that gauge is on)
+        self.alt_outputs: dict[int, sp.Expr] = {}
 
     def stage(self, name: str, pairs):
         self.stages.append((name, list(pairs)))
@@ -304,6 +308,15 @@ def bssn_rhs_stages() -> Stages:
                 "gt_rhs11": "gt_rhs11", "gt_rhs12": "gt_rhs12", "gt_rhs22": "gt_rhs22"}
     for n in RHS_NAMES:
         S.outputs.append((name_map.get(n, n), out[n]))
+
+    # Rochester puncture gauge, GR/bssn.py:194-209: the shift follows
+    #   b_rhs^i = xi_1 beta^j ad_j beta^i + 3/4 xi_2 Gt^i - eta beta^i
+    # (xi_k = BSSN_XI[k]) and B is not evolved (B_rhs = 0 before dissipation).
+    # Every other equation is the standard one.
+    for i in E3:
+        S.alt_outputs[RHS_NAMES.index(f"b_rhs{i}")] = P["xi1"] * adv(bn[i]) \
+            + sp.Rational(3, 4) * P["xi2"] * Gt[i] - P["eta"] * b[i]
+        S.alt_outputs[RHS_NAMES.index(f"B_rhs{i}")] = sp.Integer(0)
     return S
 
 

@@ -16,6 +16,7 @@ them:
     BSSN_ADV(f)        advection of field f: beta^d times its advective derivative
     BSSN_LAP(f)        igt^{lm} d_l d_m of field f (conformal metric components)
     BSSN_OUT(f, v)     store RHS of field f
+    BSSN_OUT_ALT(f, v) its value under the alternative (Rochester) gauge
 
 Parameters (lambda0..3, lambda_f0..1, eta) are plain variables of type R
 that must be in scope.  This is the project's own printer; it does not use or
@@ -116,13 +117,16 @@ def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=Fa
             exprs.append(sp.expand(e) if False else e)
     out_names = [n for n, _ in S.outputs]
     out_exprs = [e for _, e in S.outputs]
-    repl, reduced = sp.cse(exprs + out_exprs, symbols=sp.numbered_symbols(tmp_prefix))
+    alt = getattr(S, "alt_outputs", {})
+    alt_idx = sorted(alt)
+    repl, reduced = sp.cse(exprs + out_exprs + [alt[i] for i in alt_idx], symbols=sp.numbered_symbols(tmp_prefix))
     defs = {}
     for s, e in repl:
         defs[s] = e
     for s, e in zip(names, reduced[: len(names)]):
         defs[s] = e
-    out_reduced = reduced[len(names):]
+    out_reduced = reduced[len(names): len(names) + len(out_names)]
+    alt_reduced = dict(zip(alt_idx, reduced[len(names) + len(out_names):]))
 
     printer = _Printer()
     lines = []
@@ -176,4 +180,25 @@ def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=Fa
         lines.append(f"{indent}BSSN_OUT({oi}, {printer.doprint(e)});")
         if barrier:
             lines.append(f"{indent}BSSN_BARRIER();")
+    # alternative-gauge values of this group's outputs, in one block at the
+    # end: their extra inputs are loaded only when that gauge is on
+    alts = [oi for oi in order if oi in alt_reduced]
+    if alts:
+        outer = indent
+        indent = outer + "    "
+        saved_loaded, saved_emitted = set(loaded), set(emitted)
+        lines.append(f"{outer}if (BSSN_ALT_ON) {{")
+        for oi in alts:
+            ea = alt_reduced[oi]
+            for fs in sorted(ea.free_symbols, key=lambda x: x.name):
+                if fs in defs:
+                    emit(fs)
+            load_inputs(ea)
+            flops += count_flops(ea)
+            lines.append(f"{indent}BSSN_OUT_ALT({oi}, {printer.doprint(ea)});")
+        lines.append(f"{outer}}}")
+        loaded.clear()
+        loaded.update(saved_loaded)
+        emitted.clear()
+        emitted.update(saved_emitted)
     return Emitted("\n".join(lines) + "\n", flops, inputs, len(emitted))

@@ -67,6 +67,10 @@ int validate_params(const sgr_params *p)
         return fail(SGR_E_INVALID, "adv_mode must be SGR_ADV_CENTRED or SGR_ADV_UPWIND");
     if (p->precision != SGR_PREC_FP64 && p->precision != SGR_PREC_MIXED)
         return fail(SGR_E_INVALID, "precision must be SGR_PREC_FP64 or SGR_PREC_MIXED");
+    if (p->gauge != SGR_GAUGE_STANDARD && p->gauge != SGR_GAUGE_ROCHESTER)
+        return fail(SGR_E_INVALID, "gauge must be SGR_GAUGE_STANDARD or SGR_GAUGE_ROCHESTER");
+    if (p->eta_mode != SGR_ETA_CONST && p->eta_mode != SGR_ETA_FUNC)
+        return fail(SGR_E_INVALID, "eta_mode must be SGR_ETA_CONST or SGR_ETA_FUNC");
     return SGR_OK;
 }
 
@@ -100,6 +104,12 @@ Params device_params(const sgr_params &p)
     q.eta_r0 = p.eta_r0;
     q.eta_damping_exp = p.eta_damping_exp;
     q.ko_sigma = p.ko_sigma;
+    q.rochester = p.gauge == SGR_GAUGE_ROCHESTER;
+    q.eta_func = p.eta_mode == SGR_ETA_FUNC;
+    for (int i = 0; i < 3; i++) q.xi[i] = (double)p.xi[i];
+    q.eta_func_r0 = p.eta_func_r0;
+    q.eta_power[0] = p.eta_power[0];
+    q.eta_power[1] = p.eta_power[1];
     return q;
 }
 
@@ -338,6 +348,12 @@ void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision)
     p->ko_sigma = 0.01;
     p->adv_mode = adv_mode;
     p->precision = precision;
+    p->gauge = SGR_GAUGE_STANDARD;
+    p->eta_mode = SGR_ETA_CONST;
+    for (int i = 0; i < 3; i++) p->xi[i] = 0;
+    p->eta_func_r0 = 1.31;
+    p->eta_power[0] = 2.0;
+    p->eta_power[1] = 2.0;
 }
 
 int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, const sgr_params *params, int device)
@@ -364,6 +380,10 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         else if (tv && !strcmp(tv, "256")) p->derivs_threads = 256;
     }
     const bool fusedv = p->variant == Variant::Fused;
+    if (fusedv && (params->gauge != SGR_GAUGE_STANDARD || params->eta_mode != SGR_ETA_CONST)) {
+        delete p;
+        return fail(SGR_E_UNSUPPORTED, "the fused kernel evaluates the standard gauge with constant eta only");
+    }
     const int tdim[3] = {fusedv ? FUSED_TX : T, fusedv ? FUSED_TY : T, fusedv ? FUSED_TZ : T};
     p->nslots = fusedv ? 0 : NSLOT;
     std::vector<uint32_t> tile_block;

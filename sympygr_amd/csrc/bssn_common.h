@@ -151,6 +151,11 @@ struct Params {
     double lambda_f[2];
     double eta_const, eta_r0, eta_damping_exp;
     double ko_sigma;
+    int rochester;       // Rochester puncture gauge (shift equation, B not evolved)
+    int eta_func;        // eta from the eta function instead of the constant
+    double xi[3];        // BSSN_XI
+    double eta_func_r0;  // BSSN_ETA_R0
+    double eta_power[2]; // BSSN_ETA_POWER
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own

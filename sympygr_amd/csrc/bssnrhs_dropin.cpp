@@ -16,23 +16,21 @@ __attribute__((weak)) double ETA_CONST = 2.0;
 __attribute__((weak)) double ETA_R0 = 50.0;
 __attribute__((weak)) double ETA_DAMPING_EXP = 1.0;
 __attribute__((weak)) double KO_DISS_SIGMA = 0.01;
+// gauge-variant parameters (bssn/src/parameters.cpp:50-51,123)
+__attribute__((weak)) double BSSN_ETA_R0 = 1.31;
+__attribute__((weak)) double BSSN_ETA_POWER[2] = {2.0, 2.0};
+__attribute__((weak)) unsigned int BSSN_XI[3] = {0, 0, 0};
 }  // namespace bssn
 
 namespace {
 std::atomic<int32_t> g_adv{SGR_ADV_CENTRED};
 std::atomic<int32_t> g_prec{SGR_PREC_FP64};
 std::atomic<int> g_device{0};
-}  // namespace
+std::atomic<int32_t> g_gauge{SGR_GAUGE_STANDARD};
+std::atomic<int32_t> g_eta{SGR_ETA_CONST};
 
-extern "C" void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device)
-{
-    g_adv = adv_mode;
-    g_prec = precision;
-    g_device = device;
-}
-
-void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
-             const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
+void run(double **uzipVarsRHS, const double **uZipVars, unsigned int offset, const double *ptmin,
+         const double *ptmax, const unsigned int *sz, unsigned int bflag, const char *who)
 {
     sgr_params p;
     for (int i = 0; i < 4; i++) p.lambda[i] = bssn::BSSN_LAMBDA[i];
@@ -44,11 +42,44 @@ void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &
     p.ko_sigma = bssn::KO_DISS_SIGMA;
     p.adv_mode = g_adv;
     p.precision = g_prec;
+    p.gauge = g_gauge;
+    p.eta_mode = g_eta;
+    for (int i = 0; i < 3; i++) p.xi[i] = bssn::BSSN_XI[i];
+    p.eta_func_r0 = bssn::BSSN_ETA_R0;
+    p.eta_power[0] = bssn::BSSN_ETA_POWER[0];
+    p.eta_power[1] = bssn::BSSN_ETA_POWER[1];
     const uint32_t s[3] = {sz[0], sz[1], sz[2]};
     const int rc = sgr_bssnrhs_host(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, s, bflag, &p, g_device);
     if (rc != SGR_OK) {
         char buf[512];
         sgr_last_error(buf, sizeof buf);
-        throw std::runtime_error(std::string("bssnrhs: ") + buf);
+        throw std::runtime_error(std::string(who) + ": " + buf);
     }
+}
+}  // namespace
+
+extern "C" void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device)
+{
+    g_adv = adv_mode;
+    g_prec = precision;
+    g_device = device;
+}
+
+extern "C" void sgr_dropin_configure_gauge(int32_t gauge, int32_t eta_mode)
+{
+    g_gauge = gauge;
+    g_eta = eta_mode;
+}
+
+void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
+             const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
+{
+    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs");
+}
+
+// The staged form computes the same right-hand side (bssn/src/rhs.cpp:392-825).
+void bssnrhs_sep(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
+                 const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
+{
+    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs_sep");
 }

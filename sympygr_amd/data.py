@@ -182,6 +182,22 @@ def brill_lindquist_fill(xp, x, y, z, out, masses=(0.5, 0.5), centres=((2.0, 0.0
         out[GT[s]] = 0.0 * x + 1.0
 
 
+def puncture_perturbed_block(seed=5, n=16, amp=1e-2, pmin=(-0.8, -0.75, -0.7), pmax=None, h=0.1):
+    """perturbed_flat_block with chi and alpha multiplied by their
+    Brill-Lindquist values (punctures at (+-2,0,0), outside the block): every
+    field non-trivial, chi well below 1 (about 0.4), so the eta function
+    (1 - chi^p0)^-p1 stays moderate."""
+    if pmax is None:
+        pmax = tuple(p + (n - 1) * h for p in pmin)
+    u = perturbed_flat_block(seed=seed, n=n, amp=amp, pmin=pmin, pmax=pmax)
+    x, y, z = block_coords(_Np(), (n, n, n), pmin, pmax)
+    bl = [None] * NFIELDS
+    brill_lindquist_fill(_Np(), x, y, z, bl)
+    u[CHI] *= bl[CHI].reshape(-1)
+    u[ALPHA] *= bl[ALPHA].reshape(-1)
+    return u
+
+
 @dataclass
 class BlockList:
     """POD block table, one row per block (the batch entry point's input)."""

@@ -28,6 +28,9 @@ from sympygr_amd import data  # noqa: E402
 
 C0 = dict(seed=1, n=22, amp=1e-3, pmin=[-1.05] * 3, pmax=[1.05] * 3)
 FAKE = dict(n=13, pmin=[0.1, 0.2, 0.3], pmax=[1.3, 1.5, 1.4])
+GAUGE = dict(seed=5, n=16, amp=1e-2, pmin=[-0.8, -0.75, -0.7], pmax=[0.7, 0.75, 0.8])
+# gauge-variant parameters (BSSN_XI, BSSN_ETA_R0, BSSN_ETA_POWER)
+GAUGE_PARAMS = dict(xi=[1, 1, 1], eta_func_r0=1.31, eta_power=[2.0, 2.0])
 PARAMS = dict(ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0, lambda_=[1, 1, 1, 1], lambda_f=[1.0, 0.0])
 REF_FLAGS = "g++ -std=c++11 -O2 -ffp-contract=off -fopenmp (oracle/Makefile)"
 
@@ -48,6 +51,11 @@ def c0_input():
 def fake_input():
     n = FAKE["n"]
     return po.ref_fake_initial_data("u", (n, n, n), FAKE["pmin"], FAKE["pmax"])
+
+
+def gauge_input():
+    return data.puncture_perturbed_block(seed=GAUGE["seed"], n=GAUGE["n"], amp=GAUGE["amp"],
+                                         pmin=tuple(GAUGE["pmin"]), pmax=tuple(GAUGE["pmax"]))
 
 
 def main():
@@ -73,6 +81,24 @@ def main():
         np.save(os.path.join(HERE, f"rhs_{name}.npy"), out)
         manifest["cases"][name] = dict(input="c0" if geo is C0 else "fake", variant=var, bflag=bflag, n=n,
                                        file=f"rhs_{name}.npy", sha256=sha(out))
+    # gauge variants of the reference build and its staged bssnrhs_sep, on a
+    # 16^3 block with every field non-trivial
+    ug = gauge_input()
+    manifest["gauge_input"] = dict(GAUGE, recipe="sympygr_amd.data.puncture_perturbed_block", sha256=sha(ug))
+    manifest["gauge_params"] = GAUGE_PARAMS
+    gcases = [("gauge_standard_b63", "u", "standard", "const", False),
+              ("gauge_etafunc_b63", "u_func", "standard", "func", False),
+              ("gauge_rochester_b63", "u_roch", "rochester", "const", False),
+              ("gauge_rochester_etafunc_b63", "u_roch_func", "rochester", "func", False),
+              ("gauge_standard_staged_b63", "u", "standard", "const", True)]
+    n = GAUGE["n"]
+    for name, var, gauge, eta, staged in gcases:
+        P = po.make_params(upwind=True, gauge=gauge, eta=eta, **PARAMS, **GAUGE_PARAMS)
+        out = interior(po.ref_rhs_block(var, ug, (n, n, n), GAUGE["pmin"], GAUGE["pmax"], 63, P, staged=staged), n)
+        np.save(os.path.join(HERE, f"rhs_{name}.npy"), out)
+        manifest["cases"][name] = dict(input="gauge", variant=var, gauge=gauge, eta=eta,
+                                       entry="bssnrhs_sep" if staged else "bssnrhs", bflag=63, n=n,
+                                       file=f"rhs_{name}.npy", sha256=sha(out))
     # single-sweep hashes on the fake input's gt1 (field 13) with beta from the
     # same input, 13^3 block, h = 0.1
     st = {}
@@ -87,7 +113,7 @@ def main():
         json.dump(st, f, indent=1, sort_keys=True)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
-    print("wrote", len(cases), "rhs fixtures and", len(st), "stencil hashes")
+    print("wrote", len(cases) + len(gcases), "rhs fixtures and", len(st), "stencil hashes")
 
 
 if __name__ == "__main__":

@@ -39,6 +39,10 @@ def case_input(name):
         g = m["c0_input"]
         u = data.perturbed_flat_block(seed=g["seed"], n=g["n"], amp=g["amp"], pmin=tuple(g["pmin"]),
                                       pmax=tuple(g["pmax"]))
+    elif c["input"] == "gauge":
+        g = m["gauge_input"]
+        u = data.puncture_perturbed_block(seed=g["seed"], n=g["n"], amp=g["amp"], pmin=tuple(g["pmin"]),
+                                          pmax=tuple(g["pmax"]))
     else:
         g = m["fake_input"]
         u = np.load(os.path.join(GOLDEN, g["file"]))
@@ -50,7 +54,17 @@ def case_params(c, **kw):
     m = manifest()
     p = dict(m["params"])
     p.update(kw)
-    return p, c["variant"] == "u"
+    return p, c["variant"].startswith("u")
+
+
+def case_gauge(c):
+    """Gauge keywords of a golden case (standard gauge, constant eta unless
+    the case was made with another variant of the reference build)."""
+    kw = dict(gauge=c.get("gauge", "standard"), eta=c.get("eta", "const"))
+    if "gauge" in c:
+        g = manifest()["gauge_params"]
+        kw.update(xi=tuple(g["xi"]), eta_func_r0=g["eta_func_r0"], eta_power=tuple(g["eta_power"]))
+    return kw
 
 
 def small_octree(sizes=(16, 24)):

@@ -45,6 +45,17 @@ def test_dropin_symbol_exported():
     assert list((ctypes.c_uint * 4).in_dll(L, "_ZN4bssn11BSSN_LAMBDAE")) == [1, 1, 1, 1]
 
 
+def test_staged_dropin_and_gauge_globals_exported():
+    """bssnrhs_sep (bssn/include/rhs.h:39-42) with the same signature, and the
+    gauge-variant globals with the reference defaults (parameters.cpp:50-51,123)."""
+    L = ctypes.CDLL(sgr.LIB_PATH)
+    assert hasattr(L, "_Z11bssnrhs_sepPPdPPKdRKjS2_S2_PS4_S5_")
+    assert hasattr(L, "sgr_dropin_configure_gauge")
+    assert ctypes.c_double.in_dll(L, "_ZN4bssn11BSSN_ETA_R0E").value == 1.31
+    assert list((ctypes.c_double * 2).in_dll(L, "_ZN4bssn14BSSN_ETA_POWERE")) == [2.0, 2.0]
+    assert list((ctypes.c_uint * 3).in_dll(L, "_ZN4bssn7BSSN_XIE")) == [0, 0, 0]
+
+
 def test_params_defaults_match_reference():
     p = sgr.params(adv="centred", precision="fp64", ko_sigma=0.01)
     assert list(p.lambda_) == [1, 1, 1, 1]
@@ -96,6 +107,20 @@ def test_rejects_bad_params():
     p = sgr.params()
     p.precision = 3
     assert _plan_error(_one(), p).code == -1
+    p = sgr.params()
+    p.gauge = 2
+    assert _plan_error(_one(), p).code == -1
+    p = sgr.params()
+    p.eta_mode = -1
+    assert _plan_error(_one(), p).code == -1
+
+
+def test_gauge_defaults_and_fused_kernel_restriction(monkeypatch):
+    p = sgr.params()
+    assert (p.gauge, p.eta_mode, list(p.xi), p.eta_func_r0, list(p.eta_power)) == (0, 0, [0, 0, 0], 1.31, [2.0, 2.0])
+    monkeypatch.setenv("SGR_KERNEL", "fused")
+    e = _plan_error(_one(), sgr.params(gauge="rochester"))
+    assert e.code == -2 and "standard gauge" in str(e)
 
 
 def test_null_arguments_are_errors_not_crashes():

@@ -24,8 +24,8 @@ def test_input_set_matches_reference_derivative_lists():
     bssn/scripts/bssnrhs_*.h allocate them for: alpha, chi, beta0..2 in full
     (30), gt0..5 only through igt^{lm} d_l d_m gt_ij (6); the 72 advective
     derivatives enter only as one contraction beta^d agrad_d f per field
-    (24); 18 field values (Gt is only differentiated, the shift only
-    advects)."""
+    (24); the 24 field values (the Rochester gauge's shift equation reads
+    beta and Gt themselves)."""
     E = gen.generate(write=False)
     kinds = {}
     for k, f, i in E.inputs:
@@ -34,7 +34,7 @@ def test_input_set_matches_reference_derivative_lists():
     assert len(kinds["D2"]) == 30
     assert {f for f, _ in kinds["LAP"]} == {f"gt{s}" for s in range(6)}
     assert len(kinds["ADV"]) == 24
-    assert len(kinds["U"]) == 18
+    assert len(kinds["U"]) == 24
     assert {f for f, _ in kinds["ADV"]} == set(FIELDS)
     d2fields = {f for f, _ in kinds["D2"]}
     assert d2fields == {"alpha", "chi", "beta0", "beta1", "beta2"}

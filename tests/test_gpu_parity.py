@@ -8,7 +8,7 @@ test_oracle.py.
 import numpy as np
 import pytest
 
-from helpers import case_input, case_params, golden, interior, manifest, normwise
+from helpers import case_gauge, case_input, case_params, golden, interior, manifest, normwise
 
 pytestmark = pytest.mark.gpu
 
@@ -152,6 +152,68 @@ def test_cpp_dropin_bssnrhs_reads_bssn_globals():
        (ctypes.c_uint * 3)(*sz), ctypes.byref(bf))
     err = normwise(interior(out, sz), golden("c0_upwind_b63"))
     assert err.max() <= FP64_TOL, err
+
+
+GAUGE_CASES = ["gauge_standard_b63", "gauge_etafunc_b63", "gauge_rochester_b63", "gauge_rochester_etafunc_b63",
+               "gauge_standard_staged_b63"]
+
+
+@pytest.mark.parametrize("name", GAUGE_CASES)
+def test_gauge_variants_match_reference_fixture(name):
+    """The reference's compile-time gauge variants (eta function, Rochester
+    gauge, both) and its staged bssnrhs_sep, selected here at run time:
+    FP64 against the compiled reference, mixed against the oracle."""
+    import pyoracle as po
+    u, sz, pmin, pmax, c = case_input(name)
+    p, up = case_params(c)
+    kw = case_gauge(c)
+    print(f"case {name} {kw}", flush=True)
+    out = _run_gpu(u, sz, pmin, pmax, c["bflag"], "upwind", ko_sigma=p["ko_sigma"], **kw)
+    err = normwise(interior(out, sz), golden(name))
+    assert err.max() <= FP64_TOL, (name, err)
+    outm = _run_gpu(u, sz, pmin, pmax, c["bflag"], "upwind", precision="mixed", ko_sigma=p["ko_sigma"], **kw)
+    ref = po.rhs_block(u, sz, pmin, pmax, c["bflag"], po.make_params(upwind=True, ko_sigma=p["ko_sigma"], **kw))
+    em = normwise(interior(outm, sz), interior(ref, sz)).max()
+    print(f"{name} mixed err {em:.2e}", flush=True)
+    assert em <= 1e-6
+
+
+def test_cpp_dropin_bssnrhs_sep_and_gauge_switch():
+    """bssnrhs_sep (exact reference signature) with the Rochester gauge and
+    eta function switched on and BSSN_XI / BSSN_ETA_R0 / BSSN_ETA_POWER read
+    from the bssn:: globals, against the reference build with both switches."""
+    import ctypes
+    import sympygr_amd as sgr
+    from helpers import manifest
+    L = ctypes.CDLL(sgr.LIB_PATH)
+    PD = ctypes.POINTER(ctypes.c_double)
+    fn = getattr(L, "_Z11bssnrhs_sepPPdPPKdRKjS2_S2_PS4_S5_")
+    fn.argtypes = [ctypes.POINTER(PD), ctypes.POINTER(PD), ctypes.POINTER(ctypes.c_uint), PD, PD,
+                   ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
+    fn.restype = None
+    g = manifest()["gauge_params"]
+    ctypes.c_double.in_dll(L, "_ZN4bssn13KO_DISS_SIGMAE").value = 0.1
+    ctypes.c_double.in_dll(L, "_ZN4bssn11BSSN_ETA_R0E").value = g["eta_func_r0"]
+    pw = (ctypes.c_double * 2).in_dll(L, "_ZN4bssn14BSSN_ETA_POWERE")
+    pw[0], pw[1] = g["eta_power"]
+    xi = (ctypes.c_uint * 3).in_dll(L, "_ZN4bssn7BSSN_XIE")
+    for i in range(3):
+        xi[i] = g["xi"][i]
+    L.sgr_dropin_configure(1, 0, 0)
+    L.sgr_dropin_configure_gauge(1, 1)
+    try:
+        u, sz, pmin, pmax, c = case_input("gauge_rochester_etafunc_b63")
+        out = np.zeros_like(u)
+        pi = (PD * 24)(*[u[f].ctypes.data_as(PD) for f in range(24)])
+        po_ = (PD * 24)(*[out[f].ctypes.data_as(PD) for f in range(24)])
+        off, bf = ctypes.c_uint(0), ctypes.c_uint(63)
+        fn(po_, pi, ctypes.byref(off), (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+           (ctypes.c_uint * 3)(*sz), ctypes.byref(bf))
+        err = normwise(interior(out, sz), golden("gauge_rochester_etafunc_b63"))
+        assert err.max() <= FP64_TOL, err
+    finally:
+        L.sgr_dropin_configure_gauge(0, 0)
+        L.sgr_dropin_configure(0, 0, 0)
 
 
 @pytest.mark.parametrize("variant", ["fused", "workspace"])

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from helpers import GOLDEN, case_input, case_params, golden, interior, manifest, normwise
+from helpers import GOLDEN, case_gauge, case_input, case_params, golden, interior, manifest, normwise
 from sympygr_amd import data
 
 
@@ -26,10 +26,28 @@ def test_oracle_matches_reference_fixture(name):
     u, sz, pmin, pmax, c = case_input(name)
     p, up = case_params(c)
     P = po.make_params(upwind=up, ko_sigma=p["ko_sigma"], eta_const=p["eta_const"], eta_r0=p["eta_r0"],
-                       eta_damping_exp=p["eta_damping_exp"])
+                       eta_damping_exp=p["eta_damping_exp"], **case_gauge(c))
     out = po.rhs_block(u, sz, pmin, pmax, c["bflag"], P)
     err = normwise(interior(out, sz), golden(name))
     assert err.max() <= 1e-13, err
+
+
+def test_gauge_variants_differ_where_they_should():
+    """The reference's gauge switches change only the shift and B equations
+    (Rochester) and the B damping (eta function): every other field of the
+    four variant fixtures agrees to rounding, and the changed ones differ."""
+    std = golden("gauge_standard_b63")
+    for name, changed in (("gauge_etafunc_b63", {9, 10, 11}), ("gauge_rochester_b63", {6, 7, 8, 9, 10, 11}),
+                          ("gauge_rochester_etafunc_b63", {6, 7, 8, 9, 10, 11})):
+        g = golden(name)
+        err = normwise(g, std)
+        for f in range(24):
+            if f in changed:
+                assert err[f] > 1e-6, (name, f, err[f])
+            else:
+                assert err[f] <= 1e-14, (name, f, err[f])
+    # the staged bssnrhs_sep is the same system
+    assert normwise(golden("gauge_standard_staged_b63"), std).max() <= 1e-13
 
 
 def test_c0_input_reproduces_recorded_hash():
