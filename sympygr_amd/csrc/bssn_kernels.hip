@@ -67,17 +67,24 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
 // Sommerfeld condition, i.e. on tiles that touch a flagged face.
 __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && f <= 11) || f >= 18; }
 
+// Linear halo element e = ex + 14 (ey + 14 ez) -> its LDS index with rows of
+// 24 doubles: ex + 24 (ey + 14 ez) = e + 10 (e / 14).
+__device__ __forceinline__ int lds_halo(int e) { return e + 10 * (e / TH); }
+
 // NT threads per tile: 256 (two points per thread) or 512 (one point each).
 template <typename W, bool UPWIND, int NT>
 __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                 double *__restrict__ wsK, uint64_t SS, bool all_d1, uint32_t tile0)
 {
-    constexpr int NHALO = TH * TH * TH;            // 2744
+    constexpr int NHALO = TH * TH * TH;            // 2744 halo elements
+    // LDS rows of the halo tile are RS doubles apart: 48 dwords, so the rows
+    // a wave reads in one LDS pass (8 x-values each) fall on disjoint banks
+    constexpr int RS = 24, PS = RS * TH;
     constexpr int NPF = (NHALO + NT - 1) / NT;     // prefetch registers per thread
     constexpr int PPT = TILE_PTS / NT;             // points per thread
     constexpr int ZS = NT / 64;                    // z-planes per pass over the tile
-    __shared__ double U[NHALO];          // x + 14 (y + 14 z), halo coordinates
+    __shared__ double U[PS * TH];        // x + RS (y + 14 z), halo coordinates
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
 
@@ -126,7 +133,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
         const double *__restrict__ s0 = in.p[0] + B.offset;
 #pragma unroll
         for (int q = 0; q < NPF; q++)
-            if (t + NT * q < NHALO) U[t + NT * q] = src[q] >= 0 ? s0[src[q]] : 0.0;
+            if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
     }
     __syncthreads();
 
@@ -143,11 +150,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
 #pragma unroll
         for (int p = 0; p < PPT; p++) {
             if (!valid[p]) continue;
-            const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
+            const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
             if (need_d1 || (!UPWIND && !all_d1)) {
                 const double g0 = d1(c, 1, pi, n0, lo0, hi0, a0);
-                const double g1 = d1(c, TH, pj, n1, lo1, hi1, a1);
-                const double g2 = d1(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                const double g1 = d1(c, RS, pj, n1, lo1, hi1, a1);
+                const double g2 = d1(c, PS, pk[p], n2, lo2, hi2, a2);
                 if (need_d1) {
                     ws_put(&wsb[slot_d1(f, 0) * SS + wp[p]], (W)(g0));
                     ws_put(&wsb[slot_d1(f, 1) * SS + wp[p]], (W)(g1));
@@ -161,15 +168,15 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
             if (UPWIND && !all_d1) {
                 SGR_EXACT
                 const double q0 = adv(c, 1, pi, n0, lo0, hi0, bv[p][0] > 0.0 ? 1.0 : 0.0, a0);
-                const double q1 = adv(c, TH, pj, n1, lo1, hi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
-                const double q2 = adv(c, TH * TH, pk[p], n2, lo2, hi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
+                const double q1 = adv(c, RS, pj, n1, lo1, hi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
+                const double q2 = adv(c, PS, pk[p], n2, lo2, hi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
                 ws_put(&wsb[slot_adv(f) * SS + wp[p]], (W)(((bv[p][0] * q0 + bv[p][1] * q1) + bv[p][2] * q2)));
             }
             {
                 SGR_EXACT
                 const double kx = ko(c, 1, pi, n0, lo0, hi0, a0);
-                const double ky = ko(c, TH, pj, n1, lo1, hi1, a1);
-                const double kz = ko(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                const double ky = ko(c, RS, pj, n1, lo1, hi1, a1);
+                const double kz = ko(c, PS, pk[p], n2, lo2, hi2, a2);
                 ws_put(&kb[f * SS + wp[p]], kx + ky + kz);
             }
         }
@@ -180,10 +187,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
 #pragma unroll
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
-                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
+                    const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     ws_put(&wsb[slot_d2(f, 0) * SS + wp[p]], (W)(d2(c, 1, pi, n0, lo0, hi0, a0)));
-                    ws_put(&wsb[slot_d2(f, 3) * SS + wp[p]], (W)(d2(c, TH, pj, n1, lo1, hi1, a1)));
-                    ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, TH * TH, pk[p], n2, lo2, hi2, a2)));
+                    ws_put(&wsb[slot_d2(f, 3) * SS + wp[p]], (W)(d2(c, RS, pj, n1, lo1, hi1, a1)));
+                    ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, PS, pk[p], n2, lo2, hi2, a2)));
                 }
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 && gk < n2)
-                    v = d1(&U[(x + GW) + TH * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
+                    v = d1(&U[(x + GW) + RS * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
                 DX[e] = v;
             }
             for (int e = t; e < T * T * TH; e += NT) {
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 - GW && gk < n2)
-                    v = d1(&U[(x + GW) + TH * ((y + GW) + TH * z)], TH, gj, n1, lo1, hi1, a1);
+                    v = d1(&U[(x + GW) + RS * ((y + GW) + TH * z)], RS, gj, n1, lo1, hi1, a1);
                 DY[e] = v;
             }
             __syncthreads();
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
 #pragma unroll 1
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
-                    const double *c = &U[(lx + GW) + TH * ((ly + GW) + TH * (lz + ZS * p + GW))];
+                    const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
                     const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
@@ -231,9 +238,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                         double L = ig[0] * d2(c, 1, pi, n0, lo0, hi0, a0);
                         L = L + 2.0 * (ig[1] * d1(cx, T, pj, n1, lo1, hi1, a1));
                         L = L + 2.0 * (ig[2] * d1(cx, T * TH, pk[p], n2, lo2, hi2, a2));
-                        L = L + ig[3] * d2(c, TH, pj, n1, lo1, hi1, a1);
+                        L = L + ig[3] * d2(c, RS, pj, n1, lo1, hi1, a1);
                         L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, lo2, hi2, a2));
-                        L = L + ig[5] * d2(c, TH * TH, pk[p], n2, lo2, hi2, a2);
+                        L = L + ig[5] * d2(c, PS, pk[p], n2, lo2, hi2, a2);
                         ws_put(&wsb[slot_lap(f) * SS + wp[p]], (W)(L));
                     }
                 }
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
         if (f + 1 < NF) {
 #pragma unroll
             for (int q = 0; q < NPF; q++)
-                if (t + NT * q < NHALO) U[t + NT * q] = pf[q];
+                if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = pf[q];
         }
         __syncthreads();
     }
