@@ -99,6 +99,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     // does this tile contain a point of a flagged face? (uniform)
     const bool face_tile = (lo0 && tc.i0 == GW) || (hi0 && tc.i0 + T >= n0 - GW) || (lo1 && tc.j0 == GW) ||
                            (hi1 && tc.j0 + T >= n1 - GW) || (lo2 && tc.k0 == GW) || (hi2 && tc.k0 + T >= n2 - GW);
+    // face closures apply to the three points next to a flagged face: only
+    // tiles that reach them take the per-point closure branches (uniform)
+    const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + T - 1 >= n0 - GW - 3;
+    const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + T - 1 >= n1 - GW - 3;
+    const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + T - 1 >= n2 - GW - 3;
 
     const int t = threadIdx.x;
     const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;  // lz in [0,ZS); points lz + ZS p
@@ -137,6 +142,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     }
     __syncthreads();
 
+#pragma unroll 1
     for (int f = 0; f < NF; f++) {
         // fetch the next field's halo tile into registers; it lands in LDS
         // after this field's stencils, so the HBM latency overlaps them
@@ -152,9 +158,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
             if (!valid[p]) continue;
             const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
             if (need_d1 || (!UPWIND && !all_d1)) {
-                const double g0 = d1(c, 1, pi, n0, lo0, hi0, a0);
-                const double g1 = d1(c, RS, pj, n1, lo1, hi1, a1);
-                const double g2 = d1(c, PS, pk[p], n2, lo2, hi2, a2);
+                const double g0 = d1(c, 1, pi, n0, clo0, chi0, a0);
+                const double g1 = d1(c, RS, pj, n1, clo1, chi1, a1);
+                const double g2 = d1(c, PS, pk[p], n2, clo2, chi2, a2);
                 if (need_d1) {
                     ws_put(&wsb[slot_d1(f, 0) * SS + wp[p]], (W)(g0));
                     ws_put(&wsb[slot_d1(f, 1) * SS + wp[p]], (W)(g1));
@@ -167,16 +173,16 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
             }
             if (UPWIND && !all_d1) {
                 SGR_EXACT
-                const double q0 = adv(c, 1, pi, n0, lo0, hi0, bv[p][0] > 0.0 ? 1.0 : 0.0, a0);
-                const double q1 = adv(c, RS, pj, n1, lo1, hi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
-                const double q2 = adv(c, PS, pk[p], n2, lo2, hi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
+                const double q0 = adv(c, 1, pi, n0, clo0, chi0, bv[p][0] > 0.0 ? 1.0 : 0.0, a0);
+                const double q1 = adv(c, RS, pj, n1, clo1, chi1, bv[p][1] > 0.0 ? 1.0 : 0.0, a1);
+                const double q2 = adv(c, PS, pk[p], n2, clo2, chi2, bv[p][2] > 0.0 ? 1.0 : 0.0, a2);
                 ws_put(&wsb[slot_adv(f) * SS + wp[p]], (W)(((bv[p][0] * q0 + bv[p][1] * q1) + bv[p][2] * q2)));
             }
             {
                 SGR_EXACT
-                const double kx = ko(c, 1, pi, n0, lo0, hi0, a0);
-                const double ky = ko(c, RS, pj, n1, lo1, hi1, a1);
-                const double kz = ko(c, PS, pk[p], n2, lo2, hi2, a2);
+                const double kx = ko(c, 1, pi, n0, clo0, chi0, a0);
+                const double ky = ko(c, RS, pj, n1, clo1, chi1, a1);
+                const double kz = ko(c, PS, pk[p], n2, clo2, chi2, a2);
                 ws_put(&kb[f * SS + wp[p]], kx + ky + kz);
             }
         }
@@ -188,9 +194,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
                     const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
-                    ws_put(&wsb[slot_d2(f, 0) * SS + wp[p]], (W)(d2(c, 1, pi, n0, lo0, hi0, a0)));
-                    ws_put(&wsb[slot_d2(f, 3) * SS + wp[p]], (W)(d2(c, RS, pj, n1, lo1, hi1, a1)));
-                    ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, PS, pk[p], n2, lo2, hi2, a2)));
+                    ws_put(&wsb[slot_d2(f, 0) * SS + wp[p]], (W)(d2(c, 1, pi, n0, clo0, chi0, a0)));
+                    ws_put(&wsb[slot_d2(f, 3) * SS + wp[p]], (W)(d2(c, RS, pj, n1, clo1, chi1, a1)));
+                    ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, PS, pk[p], n2, clo2, chi2, a2)));
                 }
             }
             // inner derivatives of the mixed terms, valid wherever the block has them
@@ -199,7 +205,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 && gk < n2)
-                    v = d1(&U[(x + GW) + RS * (y + TH * z)], 1, gi, n0, lo0, hi0, a0);
+                    v = d1(&U[(x + GW) + RS * (y + TH * z)], 1, gi, n0, clo0, chi0, a0);
                 DX[e] = v;
             }
             for (int e = t; e < T * T * TH; e += NT) {
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 - GW && gk < n2)
-                    v = d1(&U[(x + GW) + RS * ((y + GW) + TH * z)], RS, gj, n1, lo1, hi1, a1);
+                    v = d1(&U[(x + GW) + RS * ((y + GW) + TH * z)], RS, gj, n1, clo1, chi1, a1);
                 DY[e] = v;
             }
             __syncthreads();
@@ -217,9 +223,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     if (!valid[p]) continue;
                     const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
-                    ws_put(&wsb[slot_d2(f, 1) * SS + wp[p]], (W)(d1(cx, T, pj, n1, lo1, hi1, a1)));
-                    ws_put(&wsb[slot_d2(f, 2) * SS + wp[p]], (W)(d1(cx, T * TH, pk[p], n2, lo2, hi2, a2)));
-                    ws_put(&wsb[slot_d2(f, 4) * SS + wp[p]], (W)(d1(cy, T * T, pk[p], n2, lo2, hi2, a2)));
+                    ws_put(&wsb[slot_d2(f, 1) * SS + wp[p]], (W)(d1(cx, T, pj, n1, clo1, chi1, a1)));
+                    ws_put(&wsb[slot_d2(f, 2) * SS + wp[p]], (W)(d1(cx, T * TH, pk[p], n2, clo2, chi2, a2)));
+                    ws_put(&wsb[slot_d2(f, 4) * SS + wp[p]], (W)(d1(cy, T * T, pk[p], n2, clo2, chi2, a2)));
                 }
             } else {
 #pragma unroll 1
@@ -235,12 +241,12 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     inverse_metric(g, ig);
                     {  // metric_contract(ig, h) with each h consumed as it is made
                         SGR_EXACT
-                        double L = ig[0] * d2(c, 1, pi, n0, lo0, hi0, a0);
-                        L = L + 2.0 * (ig[1] * d1(cx, T, pj, n1, lo1, hi1, a1));
-                        L = L + 2.0 * (ig[2] * d1(cx, T * TH, pk[p], n2, lo2, hi2, a2));
-                        L = L + ig[3] * d2(c, RS, pj, n1, lo1, hi1, a1);
-                        L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, lo2, hi2, a2));
-                        L = L + ig[5] * d2(c, PS, pk[p], n2, lo2, hi2, a2);
+                        double L = ig[0] * d2(c, 1, pi, n0, clo0, chi0, a0);
+                        L = L + 2.0 * (ig[1] * d1(cx, T, pj, n1, clo1, chi1, a1));
+                        L = L + 2.0 * (ig[2] * d1(cx, T * TH, pk[p], n2, clo2, chi2, a2));
+                        L = L + ig[3] * d2(c, RS, pj, n1, clo1, chi1, a1);
+                        L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, clo2, chi2, a2));
+                        L = L + ig[5] * d2(c, PS, pk[p], n2, clo2, chi2, a2);
                         ws_put(&wsb[slot_lap(f) * SS + wp[p]], (W)(L));
                     }
                 }
