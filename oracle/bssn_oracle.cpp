@@ -276,6 +276,9 @@ double metric_laplacian(const double *const *in, double *const *grad2, std::ptrd
               ig[3] * grad2[3][pp]) + 2.0 * (ig[4] * grad2[4][pp])) + ig[5] * grad2[5][pp];
 }
 
+// symmetric second-derivative slot of (j, k): xx xy xz yy yz zz
+const int SYM[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+
 // igt^ij c_i c_j at one point, igt by adjugate / determinant of gt0..5.
 double inverse_metric_quadratic(const double *const *in, std::ptrdiff_t pp, double c0, double c1, double c2)
 {
@@ -337,6 +340,8 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #define BSSN_D2(f, s) W.grad2[f][s][pp]
 #define BSSN_ADV(f) ((in[6][pp] * W.agrad[f][0][pp] + in[7][pp] * W.agrad[f][1][pp]) + in[8][pp] * W.agrad[f][2][pp])
 #define BSSN_LAP(f) metric_laplacian(in, W.grad2[f], pp)
+// d_j (d_k beta^k) = (d_j d_0 beta^0 + d_j d_1 beta^1) + d_j d_2 beta^2
+#define BSSN_GDB(j) ((W.grad2[6][SYM[j][0]][pp] + W.grad2[7][SYM[j][1]][pp]) + W.grad2[8][SYM[j][2]][pp])
 // the Rochester gauge replaces the shift (6..8) and B (9..11) equations
 #define BSSN_OUT(f, v) \
     if (!(rochester && (f) >= 6 && (f) <= 11)) out[f][pp] = (v)
@@ -349,6 +354,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #undef BSSN_D2
 #undef BSSN_ADV
 #undef BSSN_LAP
+#undef BSSN_GDB
 #undef BSSN_OUT
 #undef BSSN_OUT_ALT
 #undef BSSN_ALT_ON

@@ -22,7 +22,8 @@ xx,xy,xz,yy,yz,zz as GR/dendro.py:86-98), first derivatives
 (bssn/scripts/bssnrhs_derivs*.h).  The advective derivatives ``agrad_<d>_<f>``
 appear in the reference system only as beta^d agrad_d f; that contraction is
 the input ``adv_<f>``, one per field.  The second derivatives of gt_ij appear
-only as igt^{lm} grad2_l_m_gt_ij, the input ``lap_<gt_ij>``.
+only as igt^{lm} grad2_l_m_gt_ij, the input ``lap_<gt_ij>``, and those of the
+shift only as ``lap_beta<i>`` and ``gdivb<j>`` = d_j d_k beta^k.
 """
 from __future__ import annotations
 
@@ -96,6 +97,13 @@ class Inputs:
     def lap(f: str) -> sp.Symbol:
         return sp.Symbol(f"lap_{f}")
 
+    # d_j (d_k beta^k), j = 0..2: the other form in which the shift's second
+    # derivatives enter (the Gamma-driver), evaluated beside the derivatives
+    # as (d_j d_0 beta^0 + d_j d_1 beta^1) + d_j d_2 beta^2
+    @staticmethod
+    def gdivb(j: int) -> sp.Symbol:
+        return sp.Symbol(f"gdivb{j}")
+
 
 def _sym3(prefix: str):
     return [[sp.Symbol(f"{prefix}{sym_slot(i, j)}") for j in E3] for i in E3]
@@ -116,8 +124,11 @@ class Stages:
         self.stages.append((name, list(pairs)))
 
 
-def bssn_rhs_stages() -> Stages:
-    """BSSN RHS, GR/bssn.py:57-104, as staged assignments."""
+def bssn_rhs_stages(contract_beta: bool = True) -> Stages:
+    """BSSN RHS, GR/bssn.py:57-104, as staged assignments.  contract_beta:
+    the shift's second derivatives enter as igt^{jk} d_j d_k beta^i
+    (lap_beta<i>) and d_j (d_k beta^k) (gdivb<j>), formed beside the
+    derivatives, instead of as 18 separate inputs."""
     I = Inputs()
     u = I.u
     P = I.params
@@ -292,7 +303,10 @@ def bssn_rhs_stages() -> Stages:
         e = adv(Gtn[i])
         e -= sum(CalGt[j] * d1(bn[i], j) for j in E3)
         e += sp.Rational(2, 3) * CalGt[i] * divb
-        e += sum(igt[j][k] * d2(bn[i], j, k) + igt[i][j] * d2(bn[k], j, k) * sp.Rational(1, 3) for (j, k) in E33)
+        if contract_beta:
+            e += I.lap(bn[i]) + sum(igt[i][j] * I.gdivb(j) for j in E3) * sp.Rational(1, 3)
+        else:
+            e += sum(igt[j][k] * d2(bn[i], j, k) + igt[i][j] * d2(bn[k], j, k) * sp.Rational(1, 3) for (j, k) in E33)
         e -= sum(2 * AtU[i][j] * d1("alpha", j) for j in E3)
         e += sum(2 * alpha * C2[i][j][k] * AtU[j][k] for (j, k) in E33)
         e -= sum(alpha * (3 * inv_chi * AtU[i][j] * dchi[j] + sp.Rational(4, 3) * igt[i][j] * d1("K", j)) for j in E3)

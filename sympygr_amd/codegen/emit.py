@@ -14,7 +14,8 @@ them:
     BSSN_D1(f, d)      first derivative of field f along d
     BSSN_D2(f, s)      second derivative of field f, symmetric slot s
     BSSN_ADV(f)        advection of field f: beta^d times its advective derivative
-    BSSN_LAP(f)        igt^{lm} d_l d_m of field f (conformal metric components)
+    BSSN_LAP(f)        igt^{lm} d_l d_m of field f (conformal metric, shift)
+    BSSN_GDB(j)        d_j (d_k beta^k)
     BSSN_OUT(f, v)     store RHS of field f
     BSSN_OUT_ALT(f, v) its value under the alternative (Rochester) gauge
 
@@ -35,6 +36,7 @@ _D1 = re.compile(r"^grad_(\d)_(\w+)$")
 _D2 = re.compile(r"^grad2_(\d)_(\d)_(\w+)$")
 _ADV = re.compile(r"^adv_(\w+)$")
 _LAP = re.compile(r"^lap_(\w+)$")
+_GDB = re.compile(r"^gdivb(\d)$")
 
 
 def classify_input(name: str):
@@ -53,6 +55,9 @@ def classify_input(name: str):
     m = _LAP.match(name)
     if m and m.group(1) in FIELD_INDEX:
         return ("LAP", m.group(1), None)
+    m = _GDB.match(name)
+    if m:
+        return ("GDB", "beta0", int(m.group(1)))
     return None
 
 
@@ -150,6 +155,8 @@ def emit_stages(S, tmp_prefix="t_", indent="    ", output_order=None, barrier=Fa
                 lines.append(f"{indent}const R {fs.name} = BSSN_U({fi});")
             elif kind in ("ADV", "LAP"):
                 lines.append(f"{indent}const R {fs.name} = BSSN_{kind}({fi});")
+            elif kind == "GDB":
+                lines.append(f"{indent}const R {fs.name} = BSSN_GDB({idx});")
             else:
                 lines.append(f"{indent}const R {fs.name} = BSSN_{kind}({fi}, {idx});")
 

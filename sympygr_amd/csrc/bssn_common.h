@@ -26,23 +26,32 @@ __host__ __device__ constexpr int d2field(int q)
 
 // Derivative workspace slots, one plane of NP points each:
 //   [0,72)     first derivative of field f along d: f*3+d
-//   [72,102)   second derivatives of alpha, beta0..2, chi: 72 + q*6 + s
-//              (q = d2q(f) in 0..4; s: xx xy xz yy yz zz)
-//   [102,108)  igt^{lm} d_l d_m gt_f for the six conformal-metric fields,
-//              102 + (f - 12), igt the inverse conformal metric of the point
-//   [108,132)  advection of field f, 108 + f:
+//   [72,84)    second derivatives of alpha and chi: 72 + 6*(f == chi) + s
+//              (s: xx xy xz yy yz zz)
+//   [84,93)    igt^{lm} d_l d_m f for the shift (84 + f - 6) and the
+//              conformal metric (87 + f - 12), igt the inverse conformal
+//              metric of the point
+//   [93,96)    d_j (d_k beta^k), 93 + j
+//   [96,120)   advection of field f, 96 + f:
 //              (beta^0 ad_0 f + beta^1 ad_1 f) + beta^2 ad_2 f, with ad the
 //              upwinded stencil (upwind mode) or the centred one
 constexpr int SLOT_D1 = 0;
 constexpr int SLOT_D2 = 72;
-constexpr int SLOT_LAP = 102;
-constexpr int SLOT_ADV = 108;
-constexpr int NSLOT = 132;
+constexpr int SLOT_LAP = 84;
+constexpr int SLOT_GDB = 93;
+constexpr int SLOT_ADV = 96;
+constexpr int NSLOT = 120;
 
 __host__ __device__ constexpr int slot_d1(int f, int d) { return SLOT_D1 + f * 3 + d; }
-__host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + d2q(f) * 6 + s; }
-__host__ __device__ constexpr int slot_lap(int f) { return SLOT_LAP + (f - 12); }
+__host__ __device__ constexpr int slot_d2(int f, int s) { return SLOT_D2 + (f == 1 ? 6 : 0) + s; }
+__host__ __device__ constexpr int slot_lap(int f) { return f <= 8 ? SLOT_LAP + (f - 6) : SLOT_LAP + 3 + (f - 12); }
+__host__ __device__ constexpr int slot_gdb(int j) { return SLOT_GDB + j; }
 __host__ __device__ constexpr int slot_adv(int f) { return SLOT_ADV + f; }
+// symmetric slot of (j, k): xx xy xz yy yz zz
+__host__ __device__ constexpr int sym_slot(int j, int k)
+{
+    return j > k ? sym_slot(k, j) : (j == 0 ? k : (j == 1 ? 2 + k : 5));
+}
 
 // The derivative workspace is streamed: written once by the derivative pass,
 // read once per pointwise kernel, far larger than L2.  Non-temporal accesses

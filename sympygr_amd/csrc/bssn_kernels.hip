@@ -87,6 +87,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     __shared__ double U[PS * TH];        // x + RS (y + 14 z), halo coordinates
     __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
     __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
+    __shared__ double GD[3 * TILE_PTS];  // d_j (d_k beta^k) accumulated over the shift fields
 
     const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
     const uint32_t tile = tile0 + ct;
@@ -188,8 +189,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
         }
 
         if (d2q(f) >= 0) {  // uniform over the workgroup
-            const bool metric = f >= 12;  // gt_ij: only igt^{lm} d_l d_m gt_ij is stored
-            if (!metric) {
+            // the shift and gt_ij enter only contracted: igt^{lm} d_l d_m f is
+            // stored, and for the shift d_j d_k beta^k is accumulated in GD
+            const bool contracted = f >= 6;
+            if (!contracted) {
 #pragma unroll
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 DY[e] = v;
             }
             __syncthreads();
-            if (!metric) {
+            if (!contracted) {
 #pragma unroll
                 for (int p = 0; p < PPT; p++) {
                     if (!valid[p]) continue;
@@ -239,15 +242,37 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
 #pragma unroll
                     for (int q = 0; q < 6; q++) g[q] = in.p[12 + q][gp];
                     inverse_metric(g, ig);
-                    {  // metric_contract(ig, h) with each h consumed as it is made
+                    {
                         SGR_EXACT
-                        double L = ig[0] * d2(c, 1, pi, n0, clo0, chi0, a0);
-                        L = L + 2.0 * (ig[1] * d1(cx, T, pj, n1, clo1, chi1, a1));
-                        L = L + 2.0 * (ig[2] * d1(cx, T * TH, pk[p], n2, clo2, chi2, a2));
-                        L = L + ig[3] * d2(c, RS, pj, n1, clo1, chi1, a1);
-                        L = L + 2.0 * (ig[4] * d1(cy, T * T, pk[p], n2, clo2, chi2, a2));
-                        L = L + ig[5] * d2(c, PS, pk[p], n2, clo2, chi2, a2);
+                        const double h0 = d2(c, 1, pi, n0, clo0, chi0, a0);
+                        const double h1 = d1(cx, T, pj, n1, clo1, chi1, a1);
+                        const double h2 = d1(cx, T * TH, pk[p], n2, clo2, chi2, a2);
+                        const double h3 = d2(c, RS, pj, n1, clo1, chi1, a1);
+                        const double h4 = d1(cy, T * T, pk[p], n2, clo2, chi2, a2);
+                        const double h5 = d2(c, PS, pk[p], n2, clo2, chi2, a2);
+                        // metric_contract(ig, h)
+                        const double L = ((((ig[0] * h0 + 2.0 * (ig[1] * h1)) + 2.0 * (ig[2] * h2)) + ig[3] * h3) +
+                                          2.0 * (ig[4] * h4)) + ig[5] * h5;
                         ws_put(&wsb[slot_lap(f) * SS + wp[p]], (W)(L));
+                        if (f <= 8) {  // shift beta^k, k = f - 6: d_j d_k beta^k for j = 0, 1, 2
+                            const double q0 = f == 6 ? h0 : (f == 7 ? h1 : h2);
+                            const double q1 = f == 6 ? h1 : (f == 7 ? h3 : h4);
+                            const double q2 = f == 6 ? h2 : (f == 7 ? h4 : h5);
+                            double *gd = &GD[wp[p]];
+                            if (f == 6) {
+                                gd[0] = q0;
+                                gd[TILE_PTS] = q1;
+                                gd[2 * TILE_PTS] = q2;
+                            } else if (f == 7) {
+                                gd[0] = gd[0] + q0;
+                                gd[TILE_PTS] = gd[TILE_PTS] + q1;
+                                gd[2 * TILE_PTS] = gd[2 * TILE_PTS] + q2;
+                            } else {
+                                ws_put(&wsb[slot_gdb(0) * SS + wp[p]], (W)(gd[0] + q0));
+                                ws_put(&wsb[slot_gdb(1) * SS + wp[p]], (W)(gd[TILE_PTS] + q1));
+                                ws_put(&wsb[slot_gdb(2) * SS + wp[p]], (W)(gd[2 * TILE_PTS] + q2));
+                            }
+                        }
                     }
                 }
             }
@@ -372,6 +397,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
 #define BSSN_D1(F_, D_) ((R)ws_get(&wsb[slot_d1(F_, D_) * SS]))
 #define BSSN_D2(F_, S_) ((R)ws_get(&wsb[slot_d2(F_, S_) * SS]))
 #define BSSN_LAP(F_) ((R)ws_get(&wsb[slot_lap(F_) * SS]))
+#define BSSN_GDB(J_) ((R)ws_get(&wsb[slot_gdb(J_) * SS]))
 #define BSSN_OUT(C_, V_) cout.p[C_][pp] = (V_)
 #define BSSN_BARRIER()
 #include "gen/bssn_constraints.inc"

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
 #define BSSN_D2(F_, S_) ((R)ws_get(&wsb[slot_d2(F_, S_) * SS]))
 #define BSSN_ADV(F_) ((R)ws_get(&wsb[slot_adv(F_) * SS]))
 #define BSSN_LAP(F_) ((R)ws_get(&wsb[slot_lap(F_) * SS]))
+#define BSSN_GDB(J_) ((R)ws_get(&wsb[slot_gdb(J_) * SS]))
 // boundary value on flagged faces, + sigma KO, then the RHS or RK-stage store
 #define SGR_PW_STORE(F_, V_)                                                                                  \
     do {                                                                                                    \
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
 #undef BSSN_D2
 #undef BSSN_ADV
 #undef BSSN_LAP
+#undef BSSN_GDB
 #undef BSSN_OUT
 #undef BSSN_OUT_ALT
 #undef BSSN_ALT_ON

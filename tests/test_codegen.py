@@ -21,23 +21,24 @@ def test_committed_algebra_is_up_to_date():
 
 def test_input_set_matches_reference_derivative_lists():
     """45 first derivatives and the second derivatives of the 11 fields that
-    bssn/scripts/bssnrhs_*.h allocate them for: alpha, chi, beta0..2 in full
-    (30), gt0..5 only through igt^{lm} d_l d_m gt_ij (6); the 72 advective
-    derivatives enter only as one contraction beta^d agrad_d f per field
-    (24); the 24 field values (the Rochester gauge's shift equation reads
-    beta and Gt themselves)."""
+    bssn/scripts/bssnrhs_*.h allocate them for: alpha and chi in full (12),
+    beta0..2 and gt0..5 only through igt^{lm} d_l d_m f (9) and, for the
+    shift, d_j d_k beta^k (3); the 72 advective derivatives enter only as
+    one contraction beta^d agrad_d f per field (24); the 24 field values
+    (the Rochester gauge's shift equation reads beta and Gt themselves)."""
     E = gen.generate(write=False)
     kinds = {}
     for k, f, i in E.inputs:
         kinds.setdefault(k, set()).add((f, i))
     assert len(kinds["D1"]) == 45
-    assert len(kinds["D2"]) == 30
-    assert {f for f, _ in kinds["LAP"]} == {f"gt{s}" for s in range(6)}
+    assert len(kinds["D2"]) == 12
+    assert {f for f, _ in kinds["LAP"]} == {f"gt{s}" for s in range(6)} | {"beta0", "beta1", "beta2"}
+    assert {i for _, i in kinds["GDB"]} == {0, 1, 2}
     assert len(kinds["ADV"]) == 24
     assert len(kinds["U"]) == 24
     assert {f for f, _ in kinds["ADV"]} == set(FIELDS)
     d2fields = {f for f, _ in kinds["D2"]}
-    assert d2fields == {"alpha", "chi", "beta0", "beta1", "beta2"}
+    assert d2fields == {"alpha", "chi"}
 
 
 def _substituted_outputs():
