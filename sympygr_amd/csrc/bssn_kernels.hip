@@ -18,7 +18,7 @@
 //
 //   k_pointwise_g{1,2}
 //                one thread per interior point, one kernel per output group
-//                (alpha/beta/chi/gt/K/Gt/B; At): reads its fields and
+//                (alpha/beta/chi/gt; At/K/Gt/B): reads its fields and
 //                staged derivatives, runs that group's part of the generated
 //                algebra in R (double or float), applies the Sommerfeld
 //                condition on flagged faces, adds sigma * KO in double and
