@@ -138,6 +138,8 @@ __global__ __launch_bounds__(256) void k_hbm_copy(double2 *__restrict__ dst, con
 struct GhostTable {
     std::vector<GhostPatch> patches;  // sorted by class
     uint32_t start[GHOST_CLASSES + 1] = {};
+    std::vector<GhostBox> boxes;      // patches split into kernel work units, sorted by class
+    uint32_t bstart[GHOST_CLASSES + 1] = {};
     uint64_t nodes = 0, interp_nodes = 0;
     bool complete = true;
     std::string msg;
@@ -165,6 +167,7 @@ struct sgr_plan {
     // ghost refresh table
     GhostTable ghost;
     GhostPatch *d_ghost = nullptr;
+    GhostBox *d_gbox = nullptr;
 };
 
 // Ghost patch table of a block set (host only).
@@ -183,6 +186,61 @@ struct sgr_plan {
 static int64_t ceil_div(int64_t a, int64_t b)  // b > 0
 {
     return a >= 0 ? (a + b - 1) / b : -((-a) / b);
+}
+
+// Splits patch pi (class cls) into boxes for the ghost kernels.  Copy
+// patches: at most GHOST_COPY_NODES nodes a box.  Interpolating patches:
+// pieces n[d] per axis grown greedily until every box keeps the limits of
+// k_ghost_sep (bounded with the largest box and the widest stencil span),
+// each step taking the axis whose split costs least: the per-node loads
+// NX*Ry*Rz + NY*ey*Rz + NZ*ey*ez summed over boxes, plus a fixed cost per box.
+static void split_patch(const GhostPatch &g, uint32_t pi, int cls, std::vector<GhostBox> &out)
+{
+    const int nt[3] = {cls & 1 ? 1 : 6, cls & 2 ? 1 : 6, cls & 4 ? 1 : 6};
+    int64_t n[3] = {1, 1, 1};
+    auto side = [&](const int64_t *k, int d) { return (g.e[d] + k[d] - 1) / k[d]; };
+    auto span = [&](int64_t s, int d) -> int64_t {  // upper bound of ghost_rows for s nodes
+        return nt[d] == 1 ? s : ((s - 1) * g.ht[d] + g.hs[d] - 1) / g.hs[d] + 6;
+    };
+    auto fits = [&](const int64_t *k) {
+        const int64_t s0 = side(k, 0), s1 = side(k, 1), s2 = side(k, 2);
+        if (cls == GHOST_CLASSES - 1) return s0 * s1 * s2 <= GHOST_COPY_NODES;
+        const int64_t Ry = span(s1, 1), Rz = span(s2, 2);
+        return s0 <= GHOST_BOX_MAX && s1 <= GHOST_BOX_MAX && s2 <= GHOST_BOX_MAX && Ry <= GHOST_ROWS_MAX &&
+               Rz <= GHOST_ROWS_MAX && s0 * Ry * Rz + s0 * s1 * Rz <= GHOST_LDS;
+    };
+    auto cost = [&](const int64_t *k) {
+        const int64_t s0 = side(k, 0), s1 = side(k, 1), s2 = side(k, 2);
+        const int64_t Ry = span(s1, 1), Rz = span(s2, 2);
+        const int64_t per = nt[0] * s0 * Ry * Rz + nt[1] * s0 * s1 * Rz + nt[2] * s0 * s1 * s2;
+        return k[0] * k[1] * k[2] * (per + 2048);
+    };
+    while (!fits(n)) {
+        int best = -1;
+        int64_t bc = 0;
+        for (int d = 0; d < 3; d++) {
+            if (side(n, d) <= 1) continue;
+            int64_t m[3] = {n[0], n[1], n[2]};
+            m[d]++;
+            // copies: keep x rows whole while another axis can split
+            const int64_t c = cls == GHOST_CLASSES - 1 ? (d == 0 ? 2 : 1) * (int64_t)1e12 - side(n, d) : cost(m);
+            if (best < 0 || c < bc) best = d, bc = c;
+        }
+        n[best]++;  // all sides 1 always fits, so some side exceeds 1 here
+    }
+    for (int64_t c = 0; c < n[2]; c++)
+        for (int64_t b = 0; b < n[1]; b++)
+            for (int64_t a = 0; a < n[0]; a++) {
+                const int64_t i[3] = {a, b, c};
+                GhostBox x{};
+                x.patch = (int32_t)pi;
+                for (int d = 0; d < 3; d++) {
+                    const int64_t l = g.e[d] * i[d] / n[d], h = g.e[d] * (i[d] + 1) / n[d];
+                    x.lo[d] = g.lo[d] + (int32_t)l;
+                    x.e[d] = (int32_t)(h - l);
+                }
+                if (x.e[0] > 0 && x.e[1] > 0 && x.e[2] > 0) out.push_back(x);
+            }
 }
 
 static GhostTable build_ghost_table(const sgr_block *blocks, uint32_t nb)
@@ -312,9 +370,14 @@ static GhostTable build_ghost_table(const sgr_block *blocks, uint32_t nb)
     if (!T.complete) return T;
     for (int c = 0; c < GHOST_CLASSES; c++) {
         T.start[c] = (uint32_t)T.patches.size();
-        T.patches.insert(T.patches.end(), cls[c].begin(), cls[c].end());
+        T.bstart[c] = (uint32_t)T.boxes.size();
+        for (const GhostPatch &g : cls[c]) {
+            split_patch(g, (uint32_t)T.patches.size(), c, T.boxes);
+            T.patches.push_back(g);
+        }
     }
     T.start[GHOST_CLASSES] = (uint32_t)T.patches.size();
+    T.bstart[GHOST_CLASSES] = (uint32_t)T.boxes.size();
     return T;
 }
 
@@ -330,6 +393,7 @@ static void plan_free(sgr_plan *p)
     hipFree(p->d_wsK);
     hipFree(p->d_flag);
     hipFree(p->d_ghost);
+    hipFree(p->d_gbox);
     hipSetDevice(cur);
     delete p;
 }
@@ -477,6 +541,10 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         if ((e = hipMalloc(&p->d_ghost, gb)) != hipSuccess) return bail(e, "hipMalloc(ghost table)");
         if ((e = hipMemcpy(p->d_ghost, p->ghost.patches.data(), gb, hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(ghost table)");
+        const size_t bb = p->ghost.boxes.size() * sizeof(GhostBox);
+        if ((e = hipMalloc(&p->d_gbox, bb)) != hipSuccess) return bail(e, "hipMalloc(ghost boxes)");
+        if ((e = hipMemcpy(p->d_gbox, p->ghost.boxes.data(), bb, hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy(ghost boxes)");
     }
     hipSetDevice(cur);
     *plan = p;
@@ -620,7 +688,7 @@ int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t st
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = launch_ghost_fill(o, plan->d_ghost, plan->ghost.start, stream);
+    hipError_t e = launch_ghost_fill(o, plan->d_ghost, plan->d_gbox, plan->ghost.bstart, stream);
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "ghost fill launch");
     return SGR_OK;

@@ -58,11 +58,50 @@ struct GhostPatch {
 // sorted by class; class c occupies [start[c], start[c+1]).
 constexpr int GHOST_CLASSES = 8;
 
+// Unit of work of the ghost kernels (one workgroup): the target nodes
+// lo .. lo+e-1 (padded coordinates of the target block) of patches[patch].
+// Boxes are stored sorted by their patch's class, like the patches.
+struct GhostBox {
+    int32_t patch;
+    int32_t lo[3], e[3];
+    int32_t pad;
+};
+
+// Limits of an interpolating box (classes 0..6), kept by the host when it
+// splits patches: extent per axis, source rows held per axis for y and z,
+// and the LDS doubles of the two partial-sum arrays, ex*Ry*Rz + ex*ey*Rz.
+constexpr int GHOST_BOX_MAX = 64;
+constexpr int GHOST_ROWS_MAX = 128;
+constexpr int GHOST_LDS = 4096;
+// Nodes per box of the copy class (7).
+constexpr int GHOST_COPY_NODES = 2048;
+
+// First source node of the taps of target node idx along one axis: the
+// coinciding node (nt == 1) or the start of the 6-point stencil, kept on
+// source nodes 3 .. sm-4 (interior).  Target positions lie inside the
+// source's owned box, so lat > 0.
+__host__ __device__ inline int ghost_p0(int idx, int rel, int ht, int hs, int sm, int nt)
+{
+    const int q = (rel + idx * ht) / hs;
+    if (nt == 1) return q;
+    const int p = q - 2 < 3 ? 3 : q - 2;
+    return p < sm - 9 ? p : sm - 9;
+}
+
+// Source rows an axis of a box needs: the span of its stencils (nt == 6),
+// or one row per target node (nt == 1).
+__host__ __device__ inline int ghost_rows(int lo, int e, int rel, int ht, int hs, int sm, int nt)
+{
+    if (nt == 1) return e;
+    return ghost_p0(lo + e - 1, rel, ht, hs, sm, 6) - ghost_p0(lo, rel, ht, hs, sm, 6) + 6;
+}
+
 // Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints; uses the workspace.
 hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s);
 
-hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const uint32_t start[GHOST_CLASSES + 1],
-                             hipStream_t s);
+// boxes sorted by class: class c occupies [bstart[c], bstart[c+1])
+hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes,
+                             const uint32_t bstart[GHOST_CLASSES + 1], hipStream_t s);
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);    // derivative workspace + pointwise
 hipError_t launch_fused(const LaunchArgs &a, hipStream_t s);  // fused single pass

@@ -468,65 +468,180 @@ __device__ inline int ghost_axis(int idx, int rel, int ht, int hs, double inv_hs
     return p0;
 }
 
-// One workgroup per patch; threads walk the patch's ghost nodes and refresh
-// all 24 fields in place (sources are interior nodes, targets ghost nodes).
-// Sum order sum_c wz_c (sum_b wy_b (sum_a wx_a u)), contraction off.
+// Mixed-radix counter over (i0, i1, i2), i0 fastest, advanced by a fixed
+// stride: replaces two integer divisions per item in the box loops.
+struct Idx3 {
+    int i0, i1, i2, s0, s1, s2, n0, n1;
+    __device__ Idx3(int lin, int step, int n0_, int n1_) : n0(n0_), n1(n1_)
+    {
+        i0 = lin % n0;
+        i1 = (lin / n0) % n1;
+        i2 = lin / (n0 * n1);
+        s0 = step % n0;
+        s1 = (step / n0) % n1;
+        s2 = step / (n0 * n1);
+    }
+    __device__ void next()
+    {
+        i0 += s0;
+        int c = i0 >= n0;
+        i0 -= c ? n0 : 0;
+        i1 += s1 + c;
+        c = i1 >= n1;
+        i1 -= c ? n1 : 0;
+        i2 += s2 + c;
+    }
+};
+
+// Separable refresh of one interpolating box, field by field through LDS:
+//   T1[k][j][ix] = sum_a wx_a(ix) u(px(ix)+a, row_y j, row_z k)
+//   T2[k][iy][ix] = sum_b wy_b(iy) T1[k][ry0(iy)+b][ix]
+//   u(target)     = sum_c wz_c(iz) T2[rz0(iz)+c][iy][ix]
+// Each value is formed by the same operations in the same order as the
+// per-node sum_c wz_c (sum_b wy_b (sum_a wx_a u)) with contraction off, so
+// results are bit-identical to it, while the loads per node fall from
+// NX*NY*NZ to NX*Ry*Rz/(ey*ez).  Sources are interior nodes and targets
+// ghost nodes, so boxes never read what another box writes.
 template <int NX, int NY, int NZ>
-__global__ __launch_bounds__(256) void k_ghost_fill(OutPtrs u, const GhostPatch *__restrict__ patches)
+__global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, const GhostPatch *__restrict__ patches,
+                                                   const GhostBox *__restrict__ boxes)
 {
     SGR_EXACT
-    const GhostPatch g = patches[blockIdx.x];
-    const int npts = g.e[0] * g.e[1] * g.e[2];
-    for (int q = threadIdx.x; q < npts; q += 256) {
-        const int x = g.lo[0] + q % g.e[0], y = g.lo[1] + (q / g.e[0]) % g.e[1], z = g.lo[2] + q / (g.e[0] * g.e[1]);
-        double wx[NX], wy[NY], wz[NZ];
-        const int ax = ghost_axis<NX>(x, g.rel[0], g.ht[0], g.hs[0], g.inv_hs[0], g.sm[0], wx);
-        const int ay = ghost_axis<NY>(y, g.rel[1], g.ht[1], g.hs[1], g.inv_hs[1], g.sm[1], wy);
-        const int az = ghost_axis<NZ>(z, g.rel[2], g.ht[2], g.hs[2], g.inv_hs[2], g.sm[2], wz);
-        const uint64_t src = g.src_off + ax + (uint64_t)g.sn0 * (ay + (uint64_t)g.sn1 * az);
-        const uint64_t dst = g.dst_off + x + (uint64_t)g.dn0 * (y + (uint64_t)g.dn1 * z);
-        const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
-#pragma unroll 1
-        for (int f = 0; f < NF; f++) {
-            const double *__restrict__ v = u.p[f] + src;
-            double out = 0.0;
+    __shared__ double T[GHOST_LDS];
+    __shared__ double wxs[GHOST_BOX_MAX * NX], wys[GHOST_BOX_MAX * NY], wzs[GHOST_BOX_MAX * NZ];
+    __shared__ int px[GHOST_BOX_MAX], ry0[GHOST_BOX_MAX], rz0[GHOST_BOX_MAX];
+    __shared__ int rowy[GHOST_ROWS_MAX], rowz[GHOST_ROWS_MAX];  // source y (z) node of each row
+    const GhostBox bx = boxes[blockIdx.x];
+    const GhostPatch g = patches[bx.patch];
+    const int ex = bx.e[0], ey = bx.e[1], ez = bx.e[2];
+    const int Ry = ghost_rows(bx.lo[1], ey, g.rel[1], g.ht[1], g.hs[1], g.sm[1], NY);
+    const int Rz = ghost_rows(bx.lo[2], ez, g.rel[2], g.ht[2], g.hs[2], g.sm[2], NZ);
+    const int n1 = ex * Ry * Rz, n2 = ex * ey * Rz, n3 = ex * ey * ez;
+    // the host splits patches into boxes within these limits
+    if (ex > GHOST_BOX_MAX || ey > GHOST_BOX_MAX || ez > GHOST_BOX_MAX || Ry > GHOST_ROWS_MAX ||
+        Rz > GHOST_ROWS_MAX || n1 + n2 > GHOST_LDS)
+        return;
+    const int t = threadIdx.x;
+    const int by0 = ghost_p0(bx.lo[1], g.rel[1], g.ht[1], g.hs[1], g.sm[1], NY);
+    const int bz0 = ghost_p0(bx.lo[2], g.rel[2], g.ht[2], g.hs[2], g.sm[2], NZ);
+    if (t < ex) {
+        double w[NX];
+        px[t] = ghost_axis<NX>(bx.lo[0] + t, g.rel[0], g.ht[0], g.hs[0], g.inv_hs[0], g.sm[0], w);
 #pragma unroll
-            for (int c = 0; c < NZ; c++) {
+        for (int a = 0; a < NX; a++) wxs[t * NX + a] = w[a];
+    } else if (t >= 64 && t - 64 < ey) {
+        const int iy = t - 64;
+        double w[NY];
+        const int p = ghost_axis<NY>(bx.lo[1] + iy, g.rel[1], g.ht[1], g.hs[1], g.inv_hs[1], g.sm[1], w);
+        ry0[iy] = NY == 1 ? iy : p - by0;
+#pragma unroll
+        for (int b = 0; b < NY; b++) wys[iy * NY + b] = w[b];
+    } else if (t >= 128 && t - 128 < ez) {
+        const int iz = t - 128;
+        double w[NZ];
+        const int p = ghost_axis<NZ>(bx.lo[2] + iz, g.rel[2], g.ht[2], g.hs[2], g.inv_hs[2], g.sm[2], w);
+        rz0[iz] = NZ == 1 ? iz : p - bz0;
+#pragma unroll
+        for (int c = 0; c < NZ; c++) wzs[iz * NZ + c] = w[c];
+    }
+    for (int j = t; j < Ry; j += 256)
+        rowy[j] = NY == 1 ? ghost_p0(bx.lo[1] + j, g.rel[1], g.ht[1], g.hs[1], g.sm[1], 1) : by0 + j;
+    for (int k = t; k < Rz; k += 256)
+        rowz[k] = NZ == 1 ? ghost_p0(bx.lo[2] + k, g.rel[2], g.ht[2], g.hs[2], g.sm[2], 1) : bz0 + k;
+    __syncthreads();
+    const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
+    const uint64_t dy = g.dn0, dz = (uint64_t)g.dn0 * g.dn1;
+    const uint64_t dst0 = g.dst_off + bx.lo[0] + dy * bx.lo[1] + dz * bx.lo[2];
+    double *T1 = T, *T2 = T + n1;
+#pragma unroll 1
+    for (int f = 0; f < NF; f++) {
+        const double *__restrict__ v = u.p[f] + g.src_off;
+        {
+            Idx3 it(t, 256, ex, Ry);
+            for (int i = t; i < n1; i += 256, it.next()) {
+                const double *r = v + px[it.i0] + sy * rowy[it.i1] + sz * rowz[it.i2];
+                double row = 0.0;
+#pragma unroll
+                for (int a = 0; a < NX; a++) row = row + wxs[it.i0 * NX + a] * r[a];
+                T1[i] = row;
+            }
+        }
+        __syncthreads();
+        {
+            Idx3 it(t, 256, ex, ey);
+            for (int i = t; i < n2; i += 256, it.next()) {
+                const double *r = T1 + (it.i2 * Ry + ry0[it.i1]) * ex + it.i0;
                 double plane = 0.0;
 #pragma unroll
-                for (int b = 0; b < NY; b++) {
-                    double row = 0.0;
-#pragma unroll
-                    for (int a = 0; a < NX; a++) row = row + wx[a] * v[a + b * sy + c * sz];
-                    plane = plane + wy[b] * row;
-                }
-                out = out + wz[c] * plane;
+                for (int b = 0; b < NY; b++) plane = plane + wys[it.i1 * NY + b] * r[b * ex];
+                T2[i] = plane;
             }
-            u.p[f][dst] = out;
         }
+        __syncthreads();
+        {
+            double *__restrict__ o = u.p[f] + dst0;
+            Idx3 it(t, 256, ex, ey);
+            for (int i = t; i < n3; i += 256, it.next()) {
+                const double *r = T2 + (rz0[it.i2] * ey + it.i1) * ex + it.i0;
+                double out = 0.0;
+#pragma unroll
+                for (int c = 0; c < NZ; c++) out = out + wzs[it.i2 * NZ + c] * r[c * ey * ex];
+                o[it.i0 + dy * it.i1 + dz * it.i2] = out;
+            }
+        }
+        // the next field's first pass writes only T1, whose readers finished
+        // before the barrier above; T2 is rewritten after its next barrier
+    }
+}
+
+// Boxes whose nodes coincide with source nodes on every axis: copies, written
+// as the per-node form evaluates them (0.0 + 1.0*u, which maps -0.0 to +0.0).
+__global__ __launch_bounds__(256) void k_ghost_copy(OutPtrs u, const GhostPatch *__restrict__ patches,
+                                                    const GhostBox *__restrict__ boxes)
+{
+    SGR_EXACT
+    const GhostBox bx = boxes[blockIdx.x];
+    const GhostPatch g = patches[bx.patch];
+    const int ex = bx.e[0], ey = bx.e[1], n = ex * ey * bx.e[2];
+    const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
+    const uint64_t dy = g.dn0, dz = (uint64_t)g.dn0 * g.dn1;
+    Idx3 it(threadIdx.x, 256, ex, ey);
+    for (int i = threadIdx.x; i < n; i += 256, it.next()) {
+        const int x = bx.lo[0] + it.i0, y = bx.lo[1] + it.i1, z = bx.lo[2] + it.i2;
+        const int ax = ghost_p0(x, g.rel[0], g.ht[0], g.hs[0], g.sm[0], 1);
+        const int ay = ghost_p0(y, g.rel[1], g.ht[1], g.hs[1], g.sm[1], 1);
+        const int az = ghost_p0(z, g.rel[2], g.ht[2], g.hs[2], g.sm[2], 1);
+        const uint64_t src = g.src_off + ax + sy * ay + sz * az;
+        const uint64_t dst = g.dst_off + x + dy * y + dz * z;
+#pragma unroll 4
+        for (int f = 0; f < NF; f++) u.p[f][dst] = 0.0 + 1.0 * u.p[f][src];
     }
 }
 
 template <int CLS>
-static void launch_ghost_class(OutPtrs u, const GhostPatch *patches, const uint32_t *start, hipStream_t s)
+static void launch_ghost_class(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes, const uint32_t *bstart,
+                               hipStream_t s)
 {
-    const uint32_t n = start[CLS + 1] - start[CLS];
+    const uint32_t n = bstart[CLS + 1] - bstart[CLS];
     if (n == 0) return;
     constexpr int NX = (CLS & 1) ? 1 : 6, NY = (CLS & 2) ? 1 : 6, NZ = (CLS & 4) ? 1 : 6;
-    hipLaunchKernelGGL((k_ghost_fill<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, patches + start[CLS]);
+    if (CLS == GHOST_CLASSES - 1)
+        hipLaunchKernelGGL(k_ghost_copy, dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
+    else
+        hipLaunchKernelGGL((k_ghost_sep<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
 }
 
-hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const uint32_t start[GHOST_CLASSES + 1],
-                             hipStream_t s)
+hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes,
+                             const uint32_t bstart[GHOST_CLASSES + 1], hipStream_t s)
 {
-    launch_ghost_class<0>(u, patches, start, s);
-    launch_ghost_class<1>(u, patches, start, s);
-    launch_ghost_class<2>(u, patches, start, s);
-    launch_ghost_class<3>(u, patches, start, s);
-    launch_ghost_class<4>(u, patches, start, s);
-    launch_ghost_class<5>(u, patches, start, s);
-    launch_ghost_class<6>(u, patches, start, s);
-    launch_ghost_class<7>(u, patches, start, s);
+    launch_ghost_class<0>(u, patches, boxes, bstart, s);
+    launch_ghost_class<1>(u, patches, boxes, bstart, s);
+    launch_ghost_class<2>(u, patches, boxes, bstart, s);
+    launch_ghost_class<3>(u, patches, boxes, bstart, s);
+    launch_ghost_class<4>(u, patches, boxes, bstart, s);
+    launch_ghost_class<5>(u, patches, boxes, bstart, s);
+    launch_ghost_class<6>(u, patches, boxes, bstart, s);
+    launch_ghost_class<7>(u, patches, boxes, bstart, s);
     return hipGetLastError();
 }
 
