@@ -549,10 +549,10 @@ __global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, const GhostPatch *
     for (int k = t; k < Rz; k += 256)
         rowz[k] = NZ == 1 ? ghost_p0(bx.lo[2] + k, g.rel[2], g.ht[2], g.hs[2], g.sm[2], 1) : bz0 + k;
     __syncthreads();
-    const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
     const uint64_t dy = g.dn0, dz = (uint64_t)g.dn0 * g.dn1;
     const uint64_t dst0 = g.dst_off + bx.lo[0] + dy * bx.lo[1] + dz * bx.lo[2];
     double *T1 = T, *T2 = T + n1;
+    const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
 #pragma unroll 1
     for (int f = 0; f < NF; f++) {
         const double *__restrict__ v = u.p[f] + g.src_off;
@@ -625,7 +625,7 @@ static void launch_ghost_class(OutPtrs u, const GhostPatch *patches, const Ghost
     const uint32_t n = bstart[CLS + 1] - bstart[CLS];
     if (n == 0) return;
     constexpr int NX = (CLS & 1) ? 1 : 6, NY = (CLS & 2) ? 1 : 6, NZ = (CLS & 4) ? 1 : 6;
-    if (CLS == GHOST_CLASSES - 1)
+    if constexpr (CLS == GHOST_CLASSES - 1)
         hipLaunchKernelGGL(k_ghost_copy, dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
     else
         hipLaunchKernelGGL((k_ghost_sep<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
