@@ -281,6 +281,60 @@ def octree_section(args, hev):
     return res, drift
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def reference_baseline(bl, u, target_s=10.0, nsample=8):
+    """The compiled reference bssnrhs (oracle/_ref, upwind build) on the
+    first `nsample` blocks, one call per block as a Dendro driver makes them,
+    on one host thread (the reference has no threading and shares one memory
+    pool and parameter globals across calls).  None when not built."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    if not po.ref_available("u"):
+        return None
+    L = po.ref_lib("u")
+    P = po.make_params(upwind=True, ko_sigma=0.1)
+    lam = (ctypes.c_uint * 4)(*P.lambda_)
+    lf = (ctypes.c_double * 2)(*P.lambda_f)
+    L.ref_set_gauge_params(P.eta_func_r0, (ctypes.c_double * 2)(*P.eta_power), (ctypes.c_uint * 3)(*P.xi))
+    calls = []
+    for b in range(nsample):
+        n = bl.sz[b][0] * bl.sz[b][1] * bl.sz[b][2]
+        o = bl.offsets[b]
+        hu = np.ascontiguousarray(u[:, o:o + n].cpu().numpy())
+        out = np.zeros_like(hu)
+        calls.append((po._ptrs(out), po._ptrs(hu), (ctypes.c_double * 3)(*bl.pmin[b]),
+                      (ctypes.c_double * 3)(*bl.pmax[b]), (ctypes.c_uint * 3)(*bl.sz[b]), bl.bflag[b], hu, out))
+
+    def sweep():
+        for (po_, pi_, pmin, pmax, sz, bf, _, _) in calls:
+            L.ref_bssnrhs(po_, pi_, 0, pmin, pmax, sz, bf, lam, lf, P.eta_const, P.eta_r0, P.eta_damping_exp,
+                          P.ko_sigma)
+    sweep()  # warm-up (the reference's memory pool allocates on first use)
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < target_s or reps < 2:
+        sweep()
+        reps += 1
+        t = time.perf_counter() - t0
+    pts = sum((bl.sz[b][0] - 6) * (bl.sz[b][1] - 6) * (bl.sz[b][2] - 6) for b in range(nsample)) * reps
+    return {"value": pts / t / 1e6, "unit": "Mpoints/s", "cores": 1, "kind": "reference",
+            "sample": f"{nsample} of the 512 blocks (first in z-y-x order, faces included), {reps} passes of one "
+                      f"bssnrhs call per block, {t:.1f} s wall; the reference built from its sources "
+                      f"(oracle/Makefile, upwind variant, g++ -O2); one thread; CPU: {_cpu_model()}"}
+
+
 def cpu_baseline(bl, u, target_s=12.0):
     """The CPU oracle (this project's C++ restatement of the reference
     bssnrhs, same structure: full-array sweeps, pointwise loop, BCs, KO) on a
@@ -309,18 +363,9 @@ def cpu_baseline(bl, u, target_s=12.0):
         reps += 1
         t = time.perf_counter() - t0
     pts = sub.interior_points() * reps
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": pts / t / 1e6, "unit": "Mpoints/s", "cores": threads, "kind": "port",
             "sample": f"{nsample} of the 512 blocks (first in z-y-x order, faces included), {reps} RHS passes, "
-                      f"{t:.1f} s wall; OpenMP over blocks; CPU: {model}"}
+                      f"{t:.1f} s wall; OpenMP over blocks; CPU: {_cpu_model()}"}
 
 
 def main():
@@ -444,9 +489,17 @@ def main():
         del src, dst
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle on a bounded sample)")
-        cpu = cpu_baseline(bl, u)
-        log(f"cpu: {cpu['value']:.3f} Mpts/s on {cpu['cores']} threads")
+        # the compiled reference when oracle/_ref holds it, with this
+        # project's multi-threaded restatement beside it; else the restatement
+        log("cpu baseline (compiled reference and oracle on bounded samples)")
+        port = cpu_baseline(bl, u)
+        log(f"cpu oracle: {port['value']:.3f} Mpts/s on {port['cores']} threads")
+        ref = reference_baseline(bl, u)
+        if ref is not None:
+            log(f"cpu reference: {ref['value']:.4f} Mpts/s on 1 thread")
+            cpu = dict(ref, port=port)
+        else:
+            cpu = port
 
     line = {
         "metric": METRIC, "value": value, "unit": "Mpoints/s", "n_gpus": 1, "steps": args.steps,
