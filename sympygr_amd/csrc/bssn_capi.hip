@@ -372,7 +372,36 @@ static GhostTable build_ghost_table(const sgr_block *blocks, uint32_t nb)
         T.start[c] = (uint32_t)T.patches.size();
         T.bstart[c] = (uint32_t)T.boxes.size();
         for (const GhostPatch &g : cls[c]) {
+            const size_t b0 = T.boxes.size();
             split_patch(g, (uint32_t)T.patches.size(), c, T.boxes);
+            // every box within the limits k_ghost_sep / k_ghost_copy hold
+            // (exact source rows), and the boxes tile the patch
+            const int nt[3] = {c & 1 ? 1 : 6, c & 2 ? 1 : 6, c & 4 ? 1 : 6};
+            uint64_t vol = 0;
+            for (size_t k = b0; k < T.boxes.size(); k++) {
+                const GhostBox &x = T.boxes[k];
+                const uint64_t v = (uint64_t)x.e[0] * x.e[1] * x.e[2];
+                vol += v;
+                bool ok = true;
+                for (int d = 0; d < 3; d++) ok &= x.lo[d] >= g.lo[d] && x.lo[d] + x.e[d] <= g.lo[d] + g.e[d];
+                if (c == GHOST_CLASSES - 1) {
+                    ok &= v <= (uint64_t)GHOST_COPY_NODES;
+                } else {
+                    const int Ry = ghost_rows(x.lo[1], x.e[1], g.rel[1], g.ht[1], g.hs[1], g.sm[1], nt[1]);
+                    const int Rz = ghost_rows(x.lo[2], x.e[2], g.rel[2], g.ht[2], g.hs[2], g.sm[2], nt[2]);
+                    ok &= x.e[0] <= GHOST_BOX_MAX && x.e[1] <= GHOST_BOX_MAX && x.e[2] <= GHOST_BOX_MAX &&
+                          Ry <= GHOST_ROWS_MAX && Rz <= GHOST_ROWS_MAX &&
+                          (int64_t)x.e[0] * Ry * Rz + (int64_t)x.e[0] * x.e[1] * Rz <= GHOST_LDS;
+                }
+                if (!ok) {
+                    incomplete("internal error: a ghost box exceeds the kernel limits");
+                    return T;
+                }
+            }
+            if (vol != (uint64_t)g.e[0] * g.e[1] * g.e[2]) {
+                incomplete("internal error: ghost boxes do not tile their patch");
+                return T;
+            }
             T.patches.push_back(g);
         }
     }

@@ -45,6 +45,15 @@ def test_table_octree_is_complete():
     assert 0 < st["interpolated"] < st["nodes"]
 
 
+def test_table_of_the_benchmark_octree():
+    """The 3032-leaf octree of the benchmark: the table is complete and every
+    work box is within the refresh kernels' limits (checked while the table
+    is built; a violation is reported as an error).  Counts pinned."""
+    bl = data.octree_blocks()
+    st = sgr.ghost_table_stats(bl)
+    assert st == {"patches": 68952, "nodes": 65162112, "interpolated": 49837515}
+
+
 def test_table_refuses_unowned_ghosts_and_overlaps():
     bl = data.uniform_tiling(nb=2, n=16, lo=-1.0, hi=1.0, faces=False)
     with pytest.raises(sgr.SgrError, match="UNSUPPORTED.*owning block"):
