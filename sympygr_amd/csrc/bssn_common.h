@@ -53,6 +53,22 @@ __host__ __device__ constexpr int sym_slot(int j, int k)
     return j > k ? sym_slot(k, j) : (j == 0 ? k : (j == 1 ? 2 + k : 5));
 }
 
+// Workspace layout.  Tile-major (1, default): slot s of point p of tile t at
+// ws[t*TILE_PTS*nslots + s*TILE_PTS + p], a tile's slots one contiguous run
+// (slot stride TILE_PTS).  Slot-major (0): ws[s*SS + t*TILE_PTS + p], SS the
+// plane stride the host passes.  The host allocation covers both.  On the
+// 512 x 32^3 RHS tile-major ran at 1065-1074 Mpts/s in five runs, slot-major
+// at 1041 and 1099 (its plane alignment varies with the allocation); groups
+// of 4-128 tiles per slot run were no faster than single tiles.
+#ifndef SGR_WS_TILE_MAJOR
+#define SGR_WS_TILE_MAJOR 1
+#endif
+__host__ __device__ constexpr uint64_t ws_tile_base(uint64_t tile, int nslots)
+{
+    return tile * (uint64_t)TILE_PTS * (SGR_WS_TILE_MAJOR ? (uint64_t)nslots : 1u);
+}
+__host__ __device__ constexpr uint64_t ws_slot_stride(uint64_t ss) { return SGR_WS_TILE_MAJOR ? TILE_PTS : ss; }
+
 // The derivative workspace is streamed: written once by the derivative pass,
 // read once per pointwise kernel, far larger than L2.  Non-temporal accesses
 // keep it from displacing the field tiles the derivative pass re-reads.

@@ -1,6 +1,6 @@
 // BSSN RHS kernels for gfx950 (CDNA4).
 //
-// Two passes over 8x8x8 interior tiles of every block (slot-major workspace):
+// Two passes over 8x8x8 interior tiles of every block (tile-major workspace):
 //
 //   k_derivs     one workgroup per tile.  Field by field, the 14^3 halo tile
 //                is staged in LDS; every thread evaluates the first, second,
@@ -113,10 +113,11 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     int pk[PPT];
     uint64_t wp[PPT];
     double bv[PPT][3];  // shift at the point
-    // slot-major workspace with a padded plane stride SS (points); this tile's
-    // 512 points are contiguous within every plane
-    W *__restrict__ wsb = ws + (uint64_t)ct * TILE_PTS;
-    double *__restrict__ kb = wsK + (uint64_t)ct * TILE_PTS;
+    // this tile's workspace: its 512 points contiguous within every slot
+    // (layout: ws_tile_base, ws_slot_stride)
+    W *__restrict__ wsb = ws + ws_tile_base(ct, NSLOT);
+    double *__restrict__ kb = wsK + ws_tile_base(ct, NF);
+    SS = ws_slot_stride(SS);
 #pragma unroll
     for (int p = 0; p < PPT; p++) {
         pk[p] = tc.k0 + lz + ZS * p;
@@ -391,7 +392,8 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
-    const W *__restrict__ wsb = ws + (uint64_t)(wg >> 1) * TILE_PTS + l;
+    const W *__restrict__ wsb = ws + ws_tile_base(wg >> 1, NSLOT) + l;
+    SS = ws_slot_stride(SS);
     typedef double R;
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)ws_get(&wsb[slot_d1(F_, D_) * SS]))

@@ -19,9 +19,10 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
     const int i = tc.i0 + lx, j = tc.j0 + ly, k = tc.k0 + lz;
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
-    // slot-major workspace, plane stride SS points
-    const W *__restrict__ wsb = ws + (uint64_t)(wg >> 1) * TILE_PTS + l;
-    const double *__restrict__ kb = wsK + (uint64_t)(wg >> 1) * TILE_PTS + l;
+    // this tile's workspace (layout: ws_tile_base, ws_slot_stride)
+    const W *__restrict__ wsb = ws + ws_tile_base(wg >> 1, NSLOT) + l;
+    const double *__restrict__ kb = wsK + ws_tile_base(wg >> 1, NF) + l;
+    SS = ws_slot_stride(SS);
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
 
     // coordinates and eta exactly as bssn/src/rhs.cpp:206-219
