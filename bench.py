@@ -281,6 +281,32 @@ def octree_section(args, hev):
     return res, drift
 
 
+def host_dropin_rate(bl, u, nsample=16, reps=3):
+    """The C++ drop-in path on HOST arrays (sgr_bssnrhs_host, as bssnrhs(...)
+    is called by a Dendro driver): one call per block over the first
+    `nsample` blocks, each staging the padded block to the GPU and copying
+    the RHS back.  Interior points per second, PCIe included."""
+    import numpy as np
+    import sympygr_amd as sgr
+    n = bl.offsets[nsample - 1] + bl.sz[nsample - 1][0] * bl.sz[nsample - 1][1] * bl.sz[nsample - 1][2]
+    hu = np.ascontiguousarray(u[:, :n].cpu().numpy())
+    out = np.zeros_like(hu)
+    prm = sgr.params(adv="upwind", precision="fp64", ko_sigma=0.1)
+
+    def sweep():
+        for b in range(nsample):
+            sgr.bssnrhs(out, hu, bl.offsets[b], bl.pmin[b], bl.pmax[b], bl.sz[b], bl.bflag[b], prm)
+    sweep()  # the first call builds the staging context
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sweep()
+    t = time.perf_counter() - t0
+    pts = sum((bl.sz[b][0] - 6) * (bl.sz[b][1] - 6) * (bl.sz[b][2] - 6) for b in range(nsample)) * reps
+    return {"value": pts / t / 1e6, "unit": "Mpoints/s", "ms_per_call": 1e3 * t / (reps * nsample),
+            "what": f"bssnrhs on host arrays (sgr_bssnrhs_host), one call per block, {nsample} blocks x {reps}, "
+                    "each staging 24 x 38^3 doubles in and out over PCIe"}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -487,6 +513,9 @@ def main():
         sgr.hbm_copy(dst, src)  # k_hbm_copy: reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
+    host = host_dropin_rate(bl, u)
+    log(f"host drop-in: {host['value']:.1f} Mpts/s, {host['ms_per_call']:.2f} ms per block call (PCIe included)")
+
     cpu = None
     if not args.no_cpu:
         # the compiled reference when oracle/_ref holds it, with this
@@ -510,7 +539,7 @@ def main():
                                "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
-        "constraint_drift": drift, "large_block": large,
+        "constraint_drift": drift, "large_block": large, "host_dropin": host,
     }
     print(json.dumps(line), flush=True)
 

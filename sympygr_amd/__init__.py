@@ -305,6 +305,16 @@ def bssnrhs(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
     return out
 
 
+def rhs_block(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
+    """sgr_rhs_block: the reference bssnrhs (bssn/include/rhs.h:33-36) on
+    device tensors u, out of shape (24, N), synchronous; reads the padded
+    block at `offset`, writes its interior RHS into `out`."""
+    check(lib().sgr_rhs_block(_field_ptrs(out), _field_ptrs(u), offset, (ctypes.c_double * 3)(*pmin),
+                              (ctypes.c_double * 3)(*pmax), (ctypes.c_uint32 * 3)(*sz), bflag,
+                              ctypes.byref(prm), device))
+    return out
+
+
 class Graph:
     """A captured RK4 step (HIP graph) of one plan and fixed buffers."""
 

@@ -879,12 +879,88 @@ int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipSt
     return SGR_OK;
 }
 
-int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
-                  const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
-                  const sgr_params *params, int device)
+}  // extern "C"
+
+// Per-block entry points (sgr_rhs_block, sgr_bssnrhs_host).  A Dendro driver
+// calls bssnrhs block after block, so the plan and the staging buffers are
+// kept between calls in one process-wide context, as the reference keeps its
+// one memory pool: a one-block plan, rebuilt only when the device, the block
+// size or the kernel-selecting parameters (advection, precision, SGR_KERNEL)
+// change; the block's offset, coordinates and bflag are re-uploaded per call.
+// Calls are serialised by a mutex.  The buffers are held until the process
+// ends.
+namespace {
+struct BlockStage {
+    std::mutex m;
+    sgr_plan *plan = nullptr;
+    int device = -1;
+    uint32_t sz[3] = {0, 0, 0};
+    int32_t adv = -1, prec = -1;
+    uint64_t cap = 0;           // points per field held by d_in, d_out, h_out
+    double *d_in = nullptr, *d_out = nullptr;
+    double *h_out = nullptr;    // pinned
+};
+BlockStage g_stage;
+
+// The plan of the staging context set up for block b (sz, offset, geometry)
+// and params on `device`; the caller holds g_stage.m and the device is current.
+int stage_plan(const sgr_block &b, const sgr_params *params, int device)
 {
-    if (!pmin || !pmax || !sz) return fail(SGR_E_INVALID, "null argument");
-    sgr_block b;
+    BlockStage &S = g_stage;
+    const char *kv = getenv("SGR_KERNEL");
+    const Variant want = kv && !strcmp(kv, "fused") ? Variant::Fused : Variant::Workspace;
+    const bool same = S.plan && S.device == device && S.sz[0] == b.sz[0] && S.sz[1] == b.sz[1] &&
+                      S.sz[2] == b.sz[2] && S.adv == params->adv_mode && S.prec == params->precision &&
+                      S.plan->variant == want;
+    if (!same) {
+        if (S.plan) sgr_plan_destroy(S.plan);
+        S.plan = nullptr;
+        const int rc = sgr_plan_create(&S.plan, &b, 1, params, device);
+        if (rc) return rc;
+        S.device = device;
+        for (int d = 0; d < 3; d++) S.sz[d] = b.sz[d];
+        S.adv = params->adv_mode;
+        S.prec = params->precision;
+        return SGR_OK;
+    }
+    // same size: new offset, coordinates, bflag and physics parameters
+    if (S.plan->variant == Variant::Fused && (params->gauge != SGR_GAUGE_STANDARD || params->eta_mode != SGR_ETA_CONST))
+        return fail(SGR_E_UNSUPPORTED, "the fused kernel evaluates the standard gauge with constant eta only");
+    BlockDesc &d = S.plan->hblocks[0];
+    d.offset = b.offset;
+    d.bflag = b.bflag;
+    for (int k = 0; k < 3; k++) {
+        d.pmin[k] = b.pmin[k];
+        d.ax[k] = axis_const(b.pmin[k], b.pmax[k], b.sz[k]);
+    }
+    S.plan->params = *params;
+    HIP_TRY(hipMemcpy(S.plan->d_blocks, &d, sizeof(BlockDesc), hipMemcpyHostToDevice), "hipMemcpy(block)");
+    return SGR_OK;
+}
+
+int stage_buffers(uint64_t n, int device)
+{
+    BlockStage &S = g_stage;
+    if (S.d_in && S.cap >= n && S.device == device) return SGR_OK;
+    hipFree(S.d_in);
+    hipFree(S.d_out);
+    hipHostFree(S.h_out);
+    S.d_in = S.d_out = S.h_out = nullptr;
+    S.cap = 0;
+    hipError_t e = hipMalloc(&S.d_in, n * NF * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&S.d_out, n * NF * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc(&S.h_out, n * NF * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "staging buffers");
+    S.cap = n;
+    return SGR_OK;
+}
+
+int block_checks(const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                 const sgr_params *params, uint64_t offset, sgr_block &b)
+{
+    if (!pmin || !pmax || !sz || !params) return fail(SGR_E_INVALID, "null argument");
+    int rc = validate_params(params);
+    if (rc) return rc;
     b.offset = offset;
     b.bflag = bflag;
     for (int d = 0; d < 3; d++) {
@@ -892,19 +968,31 @@ int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NU
         b.pmin[d] = pmin[d];
         b.pmax[d] = pmax[d];
     }
-    sgr_plan *plan = nullptr;
-    int rc = sgr_plan_create(&plan, &b, 1, params, device);
+    return validate_block(b, 0);
+}
+}  // namespace
+
+extern "C" {
+
+int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                  const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                  const sgr_params *params, int device)
+{
+    if (!out || !in) return fail(SGR_E_INVALID, "null argument");
+    sgr_block b;
+    int rc = block_checks(pmin, pmax, sz, bflag, params, offset, b);
     if (rc) return rc;
+    std::lock_guard<std::mutex> lock(g_stage.m);
     int cur = 0;
     hipGetDevice(&cur);
-    hipSetDevice(device);
-    rc = sgr_rhs(plan, out, in, nullptr);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    rc = stage_plan(b, params, device);
+    if (rc == SGR_OK) rc = sgr_rhs(g_stage.plan, out, in, nullptr);
     if (rc == SGR_OK) {
-        hipError_t e = hipDeviceSynchronize();
-        if (e != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
+        hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
     }
     hipSetDevice(cur);
-    sgr_plan_destroy(plan);
     return rc;
 }
 
@@ -912,38 +1000,36 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
                      const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
                      const sgr_params *params, int device)
 {
-    if (!out || !in || !pmin || !pmax || !sz) return fail(SGR_E_INVALID, "null argument");
+    if (!out || !in) return fail(SGR_E_INVALID, "null argument");
     for (int f = 0; f < NF; f++)
         if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
+    sgr_block b;
+    int rc = block_checks(pmin, pmax, sz, bflag, params, 0, b);  // staged at offset 0
+    if (rc) return rc;
     const uint64_t n = (uint64_t)sz[0] * sz[1] * sz[2];
+    std::lock_guard<std::mutex> lock(g_stage.m);
     int cur = 0;
     hipGetDevice(&cur);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
-    double *d_in = nullptr, *d_out = nullptr;
-    std::vector<double> h_out;
-    int rc = SGR_OK;
-    hipError_t e = hipMalloc(&d_in, n * NF * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&d_out, n * NF * sizeof(double));
-    for (int f = 0; f < NF && e == hipSuccess; f++)
-        e = hipMemcpy(d_in + f * n, in[f] + offset, n * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        rc = hip_fail(e, "staging input");
-    } else {
+    rc = stage_buffers(n, device);
+    if (rc == SGR_OK) rc = stage_plan(b, params, device);
+    hipError_t e = hipSuccess;
+    for (int f = 0; f < NF && rc == SGR_OK && e == hipSuccess; f++)
+        e = hipMemcpy(g_stage.d_in + f * n, in[f] + offset, n * sizeof(double), hipMemcpyHostToDevice);
+    if (rc == SGR_OK && e != hipSuccess) rc = hip_fail(e, "staging input");
+    if (rc == SGR_OK) {
         double *o[NF];
         const double *i[NF];
         for (int f = 0; f < NF; f++) {
-            o[f] = d_out + f * n;
-            i[f] = d_in + f * n;
+            o[f] = g_stage.d_out + f * n;
+            i[f] = g_stage.d_in + f * n;
         }
-        rc = sgr_rhs_block(o, i, 0, pmin, pmax, sz, bflag, params, device);
-        if (rc == SGR_OK) {
-            h_out.resize(n * NF);
-            e = hipMemcpy(h_out.data(), d_out, n * NF * sizeof(double), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) rc = hip_fail(e, "copying RHS back");
-        }
+        rc = sgr_rhs(g_stage.plan, o, i, nullptr);
     }
-    hipFree(d_in);
-    hipFree(d_out);
+    if (rc == SGR_OK) {
+        e = hipMemcpy(g_stage.h_out, g_stage.d_out, n * NF * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "copying RHS back");
+    }
     hipSetDevice(cur);
     if (rc) return rc;
     // only interior points are written, as the reference does
@@ -952,7 +1038,7 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
         for (uint32_t k = 3; k < nz - 3; k++)
             for (uint32_t j = 3; j < ny - 3; j++) {
                 const uint64_t row = (uint64_t)nx * (j + (uint64_t)ny * k);
-                std::memcpy(out[f] + offset + row + 3, h_out.data() + f * n + row + 3, (nx - 6) * sizeof(double));
+                std::memcpy(out[f] + offset + row + 3, g_stage.h_out + f * n + row + 3, (nx - 6) * sizeof(double));
             }
     return SGR_OK;
 }

@@ -130,6 +130,41 @@ def test_host_dropin_matches_device_path():
     np.testing.assert_array_equal(interior(got, sz), interior(dev, sz))
 
 
+def test_per_block_entries_follow_each_call():
+    """sgr_bssnrhs_host and sgr_rhs_block keep one plan and staging buffers
+    between calls: a sequence of calls that changes the block's geometry,
+    bflag, offset, size, parameters and precision equals a fresh plan per
+    call every time."""
+    import sympygr_amd as sgr
+    torch = _torch()
+    u, sz, pmin, pmax, _ = case_input("c0_upwind_b63")
+    u2, sz2, pmin2, pmax2, _ = case_input("fake_upwind_b0")
+    shifted = ([p + 0.25 for p in pmin], [p + 0.25 for p in pmax])
+    calls = [(u, sz, pmin, pmax, 63, dict(adv="upwind")),
+             (u, sz, shifted[0], shifted[1], 0, dict(adv="upwind")),
+             (u, sz, pmin, pmax, 12, dict(adv="upwind", ko_sigma=0.3, eta_const=1.5)),
+             (u2, sz2, pmin2, pmax2, 0, dict(adv="upwind")),
+             (u2, sz2, pmin2, pmax2, 63, dict(adv="centred")),
+             (u, sz, pmin, pmax, 63, dict(adv="upwind", precision="mixed")),
+             (u, sz, pmin, pmax, 63, dict(adv="upwind"))]
+    for k, (v, s, lo, hi, bf, kw) in enumerate(calls):
+        ref = _run_gpu(v, s, lo, hi, bf, kw.get("adv"), kw.get("precision", "fp64"),
+                       **{a: b for a, b in kw.items() if a not in ("adv", "precision")})
+        prm = sgr.params(**kw)
+        off = 100 * (k + 1)
+        big = np.zeros((24, v.shape[1] + off))
+        big[:, off:] = v
+        host = np.full_like(big, 3.0)
+        sgr.bssnrhs(host, big, off, lo, hi, s, bf, prm)
+        np.testing.assert_array_equal(interior(host[:, off:], s), interior(ref, s), err_msg=f"host call {k}")
+        dbig = torch.from_numpy(big).cuda()
+        dout = torch.full_like(dbig, 3.0)
+        sgr.rhs_block(dout, dbig, off, lo, hi, s, bf, prm)
+        got = dout.cpu().numpy()
+        np.testing.assert_array_equal(interior(got[:, off:], s), interior(ref, s), err_msg=f"device call {k}")
+        assert np.all(got[:, :off] == 3.0)
+
+
 def test_cpp_dropin_bssnrhs_reads_bssn_globals():
     """The C++ drop-in (exact reference signature, host arrays, parameters from
     the bssn:: globals) against the reference fixture."""
