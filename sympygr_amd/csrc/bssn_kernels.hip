@@ -113,6 +113,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     int pk[PPT];
     uint64_t wp[PPT];
     double bv[PPT][3];  // shift at the point
+    double igp[PPT][6];  // inverse conformal metric at the point, for the contracted fields
     // this tile's workspace: its 512 points contiguous within every slot
     // (layout: ws_tile_base, ws_slot_stride)
     W *__restrict__ wsb = ws + ws_tile_base(ct, NSLOT);
@@ -126,6 +127,14 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
         const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
 #pragma unroll
         for (int d = 0; d < 3; d++) bv[p][d] = valid[p] ? in.p[6 + d][gp] : 0.0;
+        {
+            // once per point instead of once per contracted field (the
+            // identity outside the interior keeps the adjugate finite there)
+            double g[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) g[q] = valid[p] ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
+            inverse_metric(g, igp[p]);
+        }
     }
     // halo element -> offset in the block (or -1 outside the padded block)
     int32_t src[NPF];
@@ -238,11 +247,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cx = &DX[lx + T * ((ly + GW) + TH * (lz + ZS * p + GW))];
                     const double *cy = &DY[lx + T * (ly + T * (lz + ZS * p + GW))];
-                    const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk[p]);
-                    double g[6], ig[6];
-#pragma unroll
-                    for (int q = 0; q < 6; q++) g[q] = in.p[12 + q][gp];
-                    inverse_metric(g, ig);
+                    const double *ig = igp[p];
                     {
                         SGR_EXACT
                         const double h0 = d2(c, 1, pi, n0, clo0, chi0, a0);
