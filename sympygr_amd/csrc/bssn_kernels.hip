@@ -212,14 +212,28 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     ws_put(&wsb[slot_d2(f, 5) * SS + wp[p]], (W)(d2(c, PS, pk[p], n2, clo2, chi2, a2)));
                 }
             }
-            // inner derivatives of the mixed terms, valid wherever the block has them
-            for (int e = t; e < T * TH * TH; e += NT) {
-                const int x = e % T, y = (e / T) % TH, z = e / (T * TH);
+            // inner derivatives of the mixed terms, valid wherever the block
+            // has them.  D_x u only where D_y and D_z read it: every y row of
+            // the tile's z planes (NA), and the tile's y rows of the z halo
+            // planes (NB); the y-halo x z-halo corners are never read.
+            constexpr int NA = T * TH * T, NB = T * T * (2 * GW);
+            for (int e = t; e < NA + NB; e += NT) {
+                int x, y, z;
+                if (e < NA) {
+                    x = e % T;
+                    y = (e / T) % TH;
+                    z = GW + e / (T * TH);
+                } else {
+                    const int r = e - NA, zz = r / (T * T);
+                    x = r % T;
+                    y = GW + (r / T) % T;
+                    z = zz < GW ? zz : zz + T;
+                }
                 const int gi = tc.i0 + x, gj = tc.j0 - GW + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 && gk < n2)
                     v = d1(&U[(x + GW) + RS * (y + TH * z)], 1, gi, n0, clo0, chi0, a0);
-                DX[e] = v;
+                DX[x + T * (y + TH * z)] = v;
             }
             for (int e = t; e < T * T * TH; e += NT) {
                 const int x = e % T, y = (e / T) % T, z = e / (T * T);
