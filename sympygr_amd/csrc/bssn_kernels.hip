@@ -164,14 +164,29 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
             for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
         }
         const bool need_d1 = all_d1 || !d1_only_for_faces(f) || face_tile;
+        // fields with second derivatives (need_d1 holds for all of them):
+        // the tile points' own D_x u and D_y u are cells of the DX and DY
+        // stages of the mixed terms, written here; same calls, same values
+        const bool stage_xy = d2q(f) >= 0;
 #pragma unroll
         for (int p = 0; p < PPT; p++) {
-            if (!valid[p]) continue;
             const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + ZS * p + GW))];
+            const int cx = lx + T * ((ly + GW) + TH * (lz + ZS * p + GW)), cy = lx + T * (ly + T * (lz + ZS * p + GW));
+            if (!valid[p]) {
+                if (stage_xy) {  // a tile point outside the interior: the stages' own conditions
+                    DX[cx] = pi < n0 - GW ? d1(c, 1, pi, n0, clo0, chi0, a0) : 0.0;
+                    DY[cy] = pi < n0 - GW && pj < n1 - GW ? d1(c, RS, pj, n1, clo1, chi1, a1) : 0.0;
+                }
+                continue;
+            }
             if (need_d1 || (!UPWIND && !all_d1)) {
                 const double g0 = d1(c, 1, pi, n0, clo0, chi0, a0);
                 const double g1 = d1(c, RS, pj, n1, clo1, chi1, a1);
                 const double g2 = d1(c, PS, pk[p], n2, clo2, chi2, a2);
+                if (stage_xy) {
+                    DX[cx] = g0;
+                    DY[cy] = g1;
+                }
                 if (need_d1) {
                     ws_put(&wsb[slot_d1(f, 0) * SS + wp[p]], (W)(g0));
                     ws_put(&wsb[slot_d1(f, 1) * SS + wp[p]], (W)(g1));
@@ -213,16 +228,19 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                 }
             }
             // inner derivatives of the mixed terms, valid wherever the block
-            // has them.  D_x u only where D_y and D_z read it: every y row of
-            // the tile's z planes (NA), and the tile's y rows of the z halo
-            // planes (NB); the y-halo x z-halo corners are never read.
-            constexpr int NA = T * TH * T, NB = T * T * (2 * GW);
+            // has them.  D_x u only where D_y and D_z read it and the tile
+            // points did not write it: the y halo rows of the tile's z planes
+            // (NA), and the tile's y rows of the z halo planes (NB); the
+            // y-halo x z-halo corners are never read.  D_y u on the z halo
+            // planes.
+            constexpr int NA = T * (2 * GW) * T, NB = T * T * (2 * GW);
             for (int e = t; e < NA + NB; e += NT) {
                 int x, y, z;
                 if (e < NA) {
+                    const int yy = (e / T) % (2 * GW);
                     x = e % T;
-                    y = (e / T) % TH;
-                    z = GW + e / (T * TH);
+                    y = yy < GW ? yy : yy + T;
+                    z = GW + e / (T * 2 * GW);
                 } else {
                     const int r = e - NA, zz = r / (T * T);
                     x = r % T;
@@ -235,13 +253,14 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
                     v = d1(&U[(x + GW) + RS * (y + TH * z)], 1, gi, n0, clo0, chi0, a0);
                 DX[x + T * (y + TH * z)] = v;
             }
-            for (int e = t; e < T * T * TH; e += NT) {
-                const int x = e % T, y = (e / T) % T, z = e / (T * T);
+            for (int e = t; e < T * T * (2 * GW); e += NT) {
+                const int zz = e / (T * T);
+                const int x = e % T, y = (e / T) % T, z = zz < GW ? zz : zz + T;
                 const int gi = tc.i0 + x, gj = tc.j0 + y, gk = tc.k0 - GW + z;
                 double v = 0.0;
                 if (gi < n0 - GW && gj < n1 - GW && gk < n2)
                     v = d1(&U[(x + GW) + RS * ((y + GW) + TH * z)], RS, gj, n1, clo1, chi1, a1);
-                DY[e] = v;
+                DY[x + T * (y + T * z)] = v;
             }
             __syncthreads();
             if (!contracted) {
