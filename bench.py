@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-array drop-in leg (PCIe-inclusive per-block calls)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-drift", action="store_true",
                     help="skip the constraint-drift comparison of FP64 and mixed on the octree")
@@ -513,8 +515,10 @@ def main():
         sgr.hbm_copy(dst, src)  # k_hbm_copy: reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
-    host = host_dropin_rate(bl, u)
-    log(f"host drop-in: {host['value']:.1f} Mpts/s, {host['ms_per_call']:.2f} ms per block call (PCIe included)")
+    host = None
+    if not (args.no_host or args.counters):
+        host = host_dropin_rate(bl, u)
+        log(f"host drop-in: {host['value']:.1f} Mpts/s, {host['ms_per_call']:.2f} ms per block call (PCIe included)")
 
     cpu = None
     if not args.no_cpu:
