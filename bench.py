@@ -471,12 +471,29 @@ def main():
     achieved_tfs = FLOPS_UPWIND * pts / (rhs_ms * 1e-3) / 1e12
     dominant = "k_pointwise" if kp >= kd else "k_derivs"
     traffic, traffic_detail = measured_traffic(pts)
+    copy_gbs = None
+    if not args.counters:
+        # for scale: the HBM rate of the library's plain copy kernel here
+        csrc = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        cdst = torch.empty_like(csrc)
+        sgr.hbm_copy(cdst, csrc)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            sgr.hbm_copy(cdst, csrc)
+        e1.record()
+        torch.cuda.synchronize()
+        copy_gbs = 10 * 2 * (1 << 30) / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        del csrc, cdst
+        log(f"copy kernel: {copy_gbs:.0f} GB/s (1 GiB read + 1 GiB written)")
     roofline = {
         "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": traffic_detail,
         "kernel": "RHS pass = k_derivs + k_pointwise (event-timed per launch)",
         "dominant_kernel": dominant, "kernel_ms": {"k_derivs": kd, "k_pointwise": kp},
         "algorithmic_bytes_per_point": alg_bytes_per_point(n),
+        "traffic_rate_GBps": traffic / (rhs_ms * 1e-3) / 1e9 if traffic else None,
+        "copy_kernel_GBps": copy_gbs,
         "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
                         "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
     }
