@@ -212,14 +212,15 @@ def octree_section(args, hev):
     import torch
     import sympygr_amd as sgr
     from sympygr_amd import data
-    obl = data.octree_blocks()
+    sizes = tuple(args.octree_sizes)
+    obl = data.octree_blocks(sizes=sizes)
     t0 = time.perf_counter()
     ou = data.fill_blocks(obl, backend="torch", device="cuda")
     torch.cuda.synchronize()
     opts = obl.interior_points()
     log(f"octree: {len(obl)} uneven blocks, {opts} interior points, filled in {time.perf_counter() - t0:.1f} s")
-    res = {"workload": "3032 octree leaves over [-16,16]^3 refined toward the punctures, interior "
-                       "16/24/32/48 (+3 ghost), Brill-Lindquist, upwind, KO sigma 0.1",
+    res = {"workload": f"{len(obl)} octree leaves over [-16,16]^3 refined toward the punctures, interior "
+                       f"{'/'.join(str(x) for x in sizes)} (+3 ghost), Brill-Lindquist, upwind, KO sigma 0.1",
            "blocks": len(obl), "interior_points": opts, "padded_points": obl.total}
     nst = max(2, args.steps // 4)
     outs = {}
@@ -406,6 +407,8 @@ def main():
     ap.add_argument("--no-drift", action="store_true",
                     help="skip the constraint-drift comparison of FP64 and mixed on the octree")
     ap.add_argument("--drift-steps", type=int, default=20)
+    ap.add_argument("--octree-sizes", type=int, nargs="+", default=[16, 24, 32, 48],
+                    help="leaf interior sizes of the octree section, drawn per leaf")
     ap.add_argument("--no-octree", action="store_true",
                     help="skip the batched uneven-octree measurements (RHS, RK4 step, drift) and the large block")
     ap.add_argument("--copy-calibration", action="store_true",
