@@ -517,7 +517,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     }
     p->ntiles = (uint32_t)tile_block.size();
     // The workspace holds at most `chunk_tiles` tiles (budget SGR_WS_GIB,
-    // default 40 GiB); larger plans run as several launches over tile
+    // default 40 GiB); larger plans run as several launches over equal tile
     // ranges.  Plane stride: the chunk's point count padded by a few KiB so
     // consecutive slot planes do not share one power-of-two alignment.
     const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
@@ -529,7 +529,9 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         uint64_t cap = (uint64_t)(gib * 1073741824.0) / (per_tile ? per_tile : 1);
         if (cap < 1) cap = 1;
         const uint64_t nt = tile_block.size();
-        p->chunk_tiles = (uint32_t)(fusedv ? nt : (nt < cap ? nt : cap));
+        // equal chunks (no short last launch)
+        const uint64_t nchunks = nt ? (nt + cap - 1) / cap : 1;
+        p->chunk_tiles = (uint32_t)(fusedv ? nt : (nt + nchunks - 1) / nchunks);
         uint64_t pad = 2048 + 64;
         const char *pv = getenv("SGR_WS_PAD");
         if (pv) pad = strtoull(pv, nullptr, 10);
