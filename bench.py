@@ -40,12 +40,44 @@ def alg_bytes_per_point(n: int) -> float:
     return 192.0 * ((n + 6) / n) ** 3 + 192.0
 
 
-# Algorithmic FP64 flops per interior point, upwind mode: stencils
+# Algorithmic flops per interior point, upwind mode: stencils
 # 2*(45*6 + 72*5 + 33*7 + 33*6 + 72*7) = 3126, KO add 24*5 = 120, pointwise
 # algebra 3917 (operation count of the reference's generated expression).
-FLOPS_UPWIND = 7163
+# In mixed precision the algebra's 3917 run in FP32, the rest in FP64.
+FLOPS_STENCIL = 3126
+FLOPS_KO_ADD = 120
+FLOPS_ALGEBRA = 3917
+FLOPS_UPWIND = FLOPS_STENCIL + FLOPS_KO_ADD + FLOPS_ALGEBRA  # 7163
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+FP32_PEAK_TFS = 157.3
+
+
+def bound_ps_per_point(n: int, precision: str):
+    """Roofline time per interior point of an n^3 block (picoseconds) for the
+    HBM roof (algorithmic bytes) and the vector-ALU roof (FP64, plus FP32 for
+    the mixed algebra), and which one binds."""
+    t_hbm = alg_bytes_per_point(n) / HBM_PEAK_GBS * 1e3
+    f64 = FLOPS_UPWIND if precision == "fp64" else FLOPS_STENCIL + FLOPS_KO_ADD
+    f32 = 0 if precision == "fp64" else FLOPS_ALGEBRA
+    t_alu = f64 / FP64_PEAK_TFS + f32 / FP32_PEAK_TFS
+    return {"hbm": t_hbm, "valu": t_alu, "binding": "hbm" if t_hbm >= t_alu else "valu"}
+
+
+def mix_roofline(size_points, seconds, precision):
+    """Roofline fraction of one RHS over a mix of block sizes
+    {n: interior points}: the bound is the sum over sizes of points times the
+    binding roof's time per point (the harmonic bound over the mix);
+    frac = bound time / measured time."""
+    t_bound = t_hbm = t_alu = 0.0
+    for n, pts in size_points.items():
+        b = bound_ps_per_point(n, precision)
+        t_bound += pts * max(b["hbm"], b["valu"]) * 1e-12
+        t_hbm += pts * b["hbm"] * 1e-12
+        t_alu += pts * b["valu"] * 1e-12
+    return {"bound_s": t_bound, "frac": t_bound / seconds, "hbm_frac": t_hbm / seconds, "valu_frac": t_alu / seconds,
+            "binding": "hbm" if t_hbm >= t_alu else "valu_fp64" + ("+fp32" if precision == "mixed" else ""),
+            "bound_mpts": sum(size_points.values()) / t_bound / 1e6}
 
 
 def measured_traffic(points):
@@ -64,6 +96,34 @@ def measured_traffic(points):
     per = {k.split("<")[0]: round(v["read_bytes"] + v["write_bytes"]) for k, v in J["kernels"].items()}
     return J["rhs_pass_bytes"], {"per_kernel_bytes": per, "measured_at": J.get("git"),
                                  "calibration": J["calibration"]}
+
+
+def measured_valu(points):
+    """FP64 and FP32 vector-ALU work of one RHS pass as counted by the
+    hardware (rocprofv3 --pmc SQ_INSTS_VALU_FLOPS_FP64 ..._FP32 and their
+    transcendental parts, per dispatch, over the pass's kernels) with the
+    kernels' durations from the same run: the utilisation of the FP64 and
+    FP32 vector peaks.  None when no measurement for this workload exists."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu_bssn_rhs_512x32cubed.json")
+    try:
+        with open(path) as f:
+            J = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if J.get("interior_points") != points:
+        return None
+    out = {"measured_at": J.get("git"), "source": os.path.relpath(path, ROOT)}
+    for prec, K in J["precisions"].items():
+        f64 = sum(k["fp64_flops"] for k in K.values())
+        f32 = sum(k["fp32_flops"] for k in K.values())
+        t = sum(k["duration_ms"] for k in K.values()) * 1e-3
+        out[prec] = {"fp64_TFs": f64 / t / 1e12, "fp64_frac": f64 / t / 1e12 / FP64_PEAK_TFS,
+                     "fp32_TFs": f32 / t / 1e12, "fp32_frac": f32 / t / 1e12 / FP32_PEAK_TFS,
+                     "fp64_flops_per_point": f64 / points, "fp32_flops_per_point": f32 / points,
+                     "per_kernel": {n: {"fp64_frac": k["fp64_flops"] / (k["duration_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                                        "fp32_frac": k["fp32_flops"] / (k["duration_ms"] * 1e-3) / 1e12 / FP32_PEAK_TFS}
+                                    for n, k in K.items()}}
+    return out
 
 
 def log(msg):
@@ -218,6 +278,10 @@ def octree_section(args, hev):
     ou = data.fill_blocks(obl, backend="torch", device="cuda")
     torch.cuda.synchronize()
     opts = obl.interior_points()
+    size_pts = {}
+    for b in range(len(obl)):
+        m = obl.sz[b][0] - 6
+        size_pts[m] = size_pts.get(m, 0) + m ** 3
     log(f"octree: {len(obl)} uneven blocks, {opts} interior points, filled in {time.perf_counter() - t0:.1f} s")
     res = {"workload": f"{len(obl)} octree leaves over [-16,16]^3 refined toward the punctures, interior "
                        f"{'/'.join(str(x) for x in sizes)} (+3 ghost), Brill-Lindquist, upwind, KO sigma 0.1",
@@ -228,7 +292,8 @@ def octree_section(args, hev):
         op = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
         outs[prec] = torch.zeros_like(ou)
         w, _, _ = timed_steps(op, outs[prec], ou, nst, 1, hev)
-        res[prec] = {"value": opts * nst / w / 1e6, "chunks": op.info().chunks, "ms_per_step": 1e3 * w / nst}
+        res[prec] = {"value": opts * nst / w / 1e6, "chunks": op.info().chunks, "ms_per_step": 1e3 * w / nst,
+                     "roofline": mix_roofline(size_pts, w / nst, prec)}
         log(f"octree {prec}: {res[prec]['value']:.1f} Mpts/s in {op.info().chunks} workspace chunks")
         del op
     err = normwise_err(outs["mixed"], outs["fp64"])
@@ -319,47 +384,66 @@ def _cpu_model():
     return ""
 
 
-def reference_baseline(bl, u, target_s=10.0, nsample=8):
-    """The compiled reference bssnrhs (oracle/_ref, upwind build) on the
-    first `nsample` blocks, one call per block as a Dendro driver makes them,
-    on one host thread (the reference has no threading and shares one memory
-    pool and parameter globals across calls).  None when not built."""
-    import ctypes
+def _sample_blocks(bl, u, nint, nface):
+    """A BlockList of the first `nint` interior-only blocks and the first
+    `nface` boundary blocks (z-y-x order) of bl, with their host data."""
+    import numpy as np
+    from sympygr_amd.data import BlockList
+    inner = [b for b in range(len(bl)) if bl.bflag[b] == 0][:nint]
+    faces = [b for b in range(len(bl)) if bl.bflag[b] != 0][:nface]
+    sub = BlockList()
+    cols = []
+    for b in inner + faces:
+        sub.add(bl.sz[b], bl.pmin[b], bl.pmax[b], bl.bflag[b])
+        n = bl.sz[b][0] * bl.sz[b][1] * bl.sz[b][2]
+        cols.append(u[:, bl.offsets[b]:bl.offsets[b] + n].cpu().numpy())
+    return sub, np.ascontiguousarray(np.concatenate(cols, axis=1)), len(inner), len(faces)
+
+
+def _host_threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def reference_baseline(bl, u, target_s=10.0):
+    """The compiled reference bssnrhs (oracle/_ref, upwind build: the
+    reference's own sources, g++ -O2) on this host, as its example driver
+    runs it (bssn/cuda_gr/src/gpuBSSNExample.cpp:267-300): an OpenMP loop over
+    blocks, one bssnrhs call per block, on every host core this process may
+    use; sample: 16 interior-only and 16 boundary blocks of the workload.
+    Beside it the same code on one thread (4 + 4 blocks).  None when the
+    reference has not been built."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     if not po.ref_available("u"):
         return None
-    L = po.ref_lib("u")
     P = po.make_params(upwind=True, ko_sigma=0.1)
-    lam = (ctypes.c_uint * 4)(*P.lambda_)
-    lf = (ctypes.c_double * 2)(*P.lambda_f)
-    L.ref_set_gauge_params(P.eta_func_r0, (ctypes.c_double * 2)(*P.eta_power), (ctypes.c_uint * 3)(*P.xi))
-    calls = []
-    for b in range(nsample):
-        n = bl.sz[b][0] * bl.sz[b][1] * bl.sz[b][2]
-        o = bl.offsets[b]
-        hu = np.ascontiguousarray(u[:, o:o + n].cpu().numpy())
+    res = {}
+    for tag, threads, nint, nface in (("all", _host_threads(), 16, 16), ("one", 1, 4, 4)):
+        sub, hu, ni, nf = _sample_blocks(bl, u, nint, nface)
         out = np.zeros_like(hu)
-        calls.append((po._ptrs(out), po._ptrs(hu), (ctypes.c_double * 3)(*bl.pmin[b]),
-                      (ctypes.c_double * 3)(*bl.pmax[b]), (ctypes.c_uint * 3)(*bl.sz[b]), bl.bflag[b], hu, out))
-
-    def sweep():
-        for (po_, pi_, pmin, pmax, sz, bf, _, _) in calls:
-            L.ref_bssnrhs(po_, pi_, 0, pmin, pmax, sz, bf, lam, lf, P.eta_const, P.eta_r0, P.eta_damping_exp,
-                          P.ko_sigma)
-    sweep()  # warm-up (the reference's memory pool allocates on first use)
-    reps, t = 0, 0.0
-    t0 = time.perf_counter()
-    while t < target_s or reps < 2:
-        sweep()
-        reps += 1
-        t = time.perf_counter() - t0
-    pts = sum((bl.sz[b][0] - 6) * (bl.sz[b][1] - 6) * (bl.sz[b][2] - 6) for b in range(nsample)) * reps
-    return {"value": pts / t / 1e6, "unit": "Mpoints/s", "cores": 1, "kind": "reference",
-            "sample": f"{nsample} of the 512 blocks (first in z-y-x order, faces included), {reps} passes of one "
-                      f"bssnrhs call per block, {t:.1f} s wall; the reference built from its sources "
-                      f"(oracle/Makefile, upwind variant, g++ -O2); one thread; CPU: {_cpu_model()}"}
+        po.ref_rhs_blocks("u", hu, sub, P, nthreads=threads, out=out)  # warm-up
+        reps, t = 0, 0.0
+        t0 = time.perf_counter()
+        while t < target_s / 2 or reps < 2:
+            po.ref_rhs_blocks("u", hu, sub, P, nthreads=threads, out=out)
+            reps += 1
+            t = time.perf_counter() - t0
+        res[tag] = {"value": sub.interior_points() * reps / t / 1e6, "threads": threads, "blocks": (ni, nf),
+                    "passes": reps, "wall_s": t}
+    a, o = res["all"], res["one"]
+    return {"value": a["value"], "unit": "Mpoints/s", "cores": a["threads"], "kind": "reference",
+            "sample": f"{a['blocks'][0]} interior-only + {a['blocks'][1]} boundary blocks of the 512 (38^3), "
+                      f"{a['passes']} passes in {a['wall_s']:.1f} s; the reference bssnrhs built from its sources "
+                      f"(oracle/Makefile, upwind variant, g++ -O2 -fopenmp), OpenMP over blocks on {a['threads']} "
+                      f"threads as bssn/cuda_gr/src/gpuBSSNExample.cpp:267-300 runs it; nproc {os.cpu_count()}, "
+                      f"CPU: {_cpu_model()}",
+            "one_thread": {"value": o["value"], "unit": "Mpoints/s",
+                           "sample": f"{o['blocks'][0]} interior + {o['blocks'][1]} boundary blocks, "
+                                     f"{o['passes']} passes in {o['wall_s']:.1f} s"}}
 
 
 def cpu_baseline(bl, u, target_s=12.0):
@@ -369,17 +453,8 @@ def cpu_baseline(bl, u, target_s=12.0):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    from sympygr_amd.data import BlockList
-    try:
-        ncpu = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncpu = os.cpu_count() or 1
-    threads = max(1, min(16, ncpu))
-    nsample = max(threads, 16)
-    sub = BlockList()
-    for b in range(nsample):
-        sub.add(bl.sz[b], bl.pmin[b], bl.pmax[b], bl.bflag[b])
-    hu = np.ascontiguousarray(u[:, : sub.total].cpu().numpy())
+    threads = _host_threads()
+    sub, hu, ni, nf = _sample_blocks(bl, u, 16, 16)
     out = np.zeros_like(hu)
     P = po.make_params(upwind=True, ko_sigma=0.1)
     po.rhs_blocks(hu, sub, P, nthreads=threads, out=out)  # warm-up
@@ -391,8 +466,8 @@ def cpu_baseline(bl, u, target_s=12.0):
         t = time.perf_counter() - t0
     pts = sub.interior_points() * reps
     return {"value": pts / t / 1e6, "unit": "Mpoints/s", "cores": threads, "kind": "port",
-            "sample": f"{nsample} of the 512 blocks (first in z-y-x order, faces included), {reps} RHS passes, "
-                      f"{t:.1f} s wall; OpenMP over blocks; CPU: {_cpu_model()}"}
+            "sample": f"{ni} interior-only + {nf} boundary blocks of the 512, {reps} RHS passes, {t:.1f} s wall; "
+                      f"OpenMP over blocks on {threads} threads; nproc {os.cpu_count()}, CPU: {_cpu_model()}"}
 
 
 def main():
@@ -406,7 +481,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-drift", action="store_true",
                     help="skip the constraint-drift comparison of FP64 and mixed on the octree")
-    ap.add_argument("--drift-steps", type=int, default=20)
+    ap.add_argument("--drift-steps", type=int, default=100)
     ap.add_argument("--octree-sizes", type=int, nargs="+", default=[16, 24, 32, 48],
                     help="leaf interior sizes of the octree section, drawn per leaf")
     ap.add_argument("--no-octree", action="store_true",
@@ -448,7 +523,8 @@ def main():
     log(f"fp64 plan: workspace {plan.info().workspace_bytes / 2**30:.1f} GiB; warmup {args.warmup}, steps {args.steps}")
     wall, kd, kp = timed_steps(plan, out, u, args.steps, args.warmup, hev)
     value = pts * args.steps / wall / 1e6
-    log(f"fp64: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_derivs {kd:.3f} ms, k_pointwise {kp:.3f} ms")
+    log(f"fp64: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_stencil {kd:.3f} ms, "
+        f"k_pointwise {kp:.3f} ms")
     ref_out = out if args.counters else out.clone()
     del plan
 
@@ -460,9 +536,11 @@ def main():
         mval = pts * args.steps / mwall / 1e6
         err = normwise_err(mout, ref_out)
         rms = ((mout - ref_out).pow(2).mean(dim=1).sqrt() / ref_out.pow(2).mean(dim=1).sqrt().clamp_min(1e-300))
-        log(f"mixed: {mval:.1f} Mpts/s, k_derivs {mkd:.3f} ms, k_pointwise {mkp:.3f} ms, max err {err.max().item():.2e}")
+        log(f"mixed: {mval:.1f} Mpts/s, k_stencil {mkd:.3f} ms, k_pointwise {mkp:.3f} ms, "
+            f"max err {err.max().item():.2e}")
         mixed = {"value": mval, "ms_per_step": 1e3 * mwall / args.steps, "speedup_vs_fp64": mval / value,
-                 "kernel_ms": {"k_derivs": mkd, "k_pointwise": mkp},
+                 "kernel_ms": {"k_stencil": mkd, "k_pointwise": mkp},
+                 "roofline": mix_roofline({n: pts}, (mkd + mkp) * 1e-3, "mixed"),
                  "max_normwise_rel_err_vs_fp64": err.max().item(),
                  "per_field_normwise_rel_err": [float(f"{x:.3e}") for x in err.tolist()],
                  "per_field_rms_rel_err": [float(f"{x:.3e}") for x in rms.tolist()]}
@@ -472,7 +550,14 @@ def main():
     abytes = alg_bytes_per_point(n) * pts
     achieved_gbs = abytes / (rhs_ms * 1e-3) / 1e9
     achieved_tfs = FLOPS_UPWIND * pts / (rhs_ms * 1e-3) / 1e12
-    dominant = "k_pointwise" if kp >= kd else "k_derivs"
+    dominant = "k_pointwise" if kp >= kd else "k_stencil"
+    # per kernel: k_stencil does the stencils and the KO sums (FP64) and
+    # reads the fields over the padded blocks; the pointwise kernels run the
+    # algebra
+    st_flops = FLOPS_STENCIL + FLOPS_KO_ADD
+    st_tfs = st_flops * pts / (kd * 1e-3) / 1e12
+    st_gbs = 192.0 * ((n + 6) / n) ** 3 * pts / (kd * 1e-3) / 1e9
+    pw_tfs = FLOPS_ALGEBRA * pts / (kp * 1e-3) / 1e12
     traffic, traffic_detail = measured_traffic(pts)
     copy_gbs = None
     if not args.counters:
@@ -489,16 +574,28 @@ def main():
         copy_gbs = 10 * 2 * (1 << 30) / (e0.elapsed_time(e1) * 1e-3) / 1e9
         del csrc, cdst
         log(f"copy kernel: {copy_gbs:.0f} GB/s (1 GiB read + 1 GiB written)")
+    bnd = bound_ps_per_point(n, "fp64")
     roofline = {
-        "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_detail": traffic_detail,
-        "kernel": "RHS pass = k_derivs + k_pointwise (event-timed per launch)",
-        "dominant_kernel": dominant, "kernel_ms": {"k_derivs": kd, "k_pointwise": kp},
-        "algorithmic_bytes_per_point": alg_bytes_per_point(n),
-        "traffic_rate_GBps": traffic / (rhs_ms * 1e-3) / 1e9 if traffic else None,
-        "copy_kernel_GBps": copy_gbs,
-        "fp64_vector": {"achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                        "frac": achieved_tfs / FP64_PEAK_TFS, "flops_per_point": FLOPS_UPWIND},
+        # at n = 32 the FP64 vector roof binds (91 ps/pt against 64 for HBM)
+        "bound": "fp64_vector" if bnd["binding"] == "valu" else "hbm",
+        "achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved_tfs / FP64_PEAK_TFS,
+        "traffic": traffic, "traffic_detail": traffic_detail,
+        "kernel": "RHS pass = k_stencil + k_pointwise_g1 + k_pointwise_g2 (HIP events around each launch on "
+                  "the launch stream)",
+        "flops_per_point": FLOPS_UPWIND, "bound_ps_per_point": bnd,
+        "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                "algorithmic_bytes_per_point": alg_bytes_per_point(n),
+                "traffic_rate_GBps": traffic / (rhs_ms * 1e-3) / 1e9 if traffic else None,
+                "copy_kernel_GBps": copy_gbs},
+        "dominant_kernel": dominant,
+        "kernels": {
+            "k_stencil": {"ms": kd, "fp64_TFs": st_tfs, "fp64_frac": st_tfs / FP64_PEAK_TFS,
+                          "field_read_GBps": st_gbs, "hbm_frac": st_gbs / HBM_PEAK_GBS,
+                          "flops_per_point": st_flops},
+            "k_pointwise": {"ms": kp, "fp64_TFs": pw_tfs, "fp64_frac": pw_tfs / FP64_PEAK_TFS,
+                            "flops_per_point": FLOPS_ALGEBRA},
+        },
+        "measured_valu": measured_valu(pts),
     }
     octree = drift = None
     if not args.no_octree:
@@ -549,7 +646,8 @@ def main():
         log(f"cpu oracle: {port['value']:.3f} Mpts/s on {port['cores']} threads")
         ref = reference_baseline(bl, u)
         if ref is not None:
-            log(f"cpu reference: {ref['value']:.4f} Mpts/s on 1 thread")
+            log(f"cpu reference: {ref['value']:.4f} Mpts/s on {ref['cores']} threads, "
+                f"{ref['one_thread']['value']:.4f} on 1")
             cpu = dict(ref, port=port)
         else:
             cpu = port

@@ -6,13 +6,18 @@
  * same bssn:: globals the reference reads (bssn/src/parameters.cpp:99-109);
  * the library carries weak default definitions that the application's own
  * parameters.cpp overrides at link time.  Arrays are HOST memory, as in the
- * reference; each call stages the block to the GPU and copies the interior
- * RHS back.  Errors throw std::runtime_error (the reference has no error
- * channel).
+ * reference; each call stages the block to the GPU (one copy from pinned
+ * memory) and copies the interior RHS back.  Calls from several host threads
+ * (an OpenMP loop over blocks, as the reference's drivers run) proceed
+ * concurrently, each thread with its own staging context and stream.  The
+ * reference has no error channel and nothing here throws: a failed call
+ * leaves NaN in the block's interior RHS, prints the first failure to stderr
+ * and keeps it for sgr_dropin_status().
  */
 #ifndef SYMPYGR_BSSN_DROPIN_H
 #define SYMPYGR_BSSN_DROPIN_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
@@ -39,6 +44,17 @@ void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device);
  * BSSN_ETA_R0, BSSN_ETA_POWER (bssn/src/parameters.cpp:50-51,123).
  * Default: standard gauge, constant eta. */
 void sgr_dropin_configure_gauge(int32_t gauge, int32_t eta_mode);
+
+/* The first failure of bssnrhs / bssnrhs_sep since the last call (an SGR_E*
+ * code, SGR_OK if none) and its message; clears it.  No reference
+ * counterpart (the reference's bssnrhs cannot fail). */
+int sgr_dropin_status(char *buf, size_t len);
+
+/* The parameters the next bssnrhs call will use, as read from the bssn::
+ * globals and the two configure calls above (sgr_params of sympygr_amd.h;
+ * passed as void * so this header needs no HIP header).  No reference
+ * counterpart. */
+void sgr_dropin_params(void *params);
 }
 
 #endif

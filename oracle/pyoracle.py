@@ -56,7 +56,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # SGR_ORACLE_LIB: another build of the same sources (the sanitizer test)
+        path = os.environ.get("SGR_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run `make -C oracle oracle`")
         L = ctypes.CDLL(path)
@@ -175,6 +176,33 @@ def ref_rhs_block(variant, u, sz, pmin, pmax, bflag, params, staged=False):
     return out
 
 
+def ref_rhs_blocks(variant, u, bl, params, nthreads=1, out=None):
+    """The compiled reference over a BlockList on `nthreads` host threads
+    (ref_bssnrhs_blocks: an OpenMP loop over blocks, one bssnrhs call each,
+    as the reference's own example driver runs it)."""
+    L = ref_lib(variant)
+    if not hasattr(L.ref_bssnrhs_blocks, "_typed"):
+        L.ref_bssnrhs_blocks.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint,
+                                         ctypes.POINTER(ctypes.c_ulonglong), P_D, P_D, ctypes.POINTER(ctypes.c_uint),
+                                         ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint), P_D,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int]
+        L.ref_bssnrhs_blocks._typed = True
+    nb = len(bl)
+    if out is None:
+        out = np.zeros_like(u)
+    L.ref_set_gauge_params(params.eta_func_r0, (ctypes.c_double * 2)(*params.eta_power),
+                           (ctypes.c_uint * 3)(*params.xi))
+    L.ref_bssnrhs_blocks(_ptrs(out), _ptrs(u), nb, (ctypes.c_ulonglong * nb)(*bl.offsets),
+                         (ctypes.c_double * (3 * nb))(*[v for p in bl.pmin for v in p]),
+                         (ctypes.c_double * (3 * nb))(*[v for p in bl.pmax for v in p]),
+                         (ctypes.c_uint * (3 * nb))(*[v for s_ in bl.sz for v in s_]),
+                         (ctypes.c_uint * nb)(*bl.bflag), (ctypes.c_uint * 4)(*params.lambda_),
+                         (ctypes.c_double * 2)(*params.lambda_f), params.eta_const, params.eta_r0,
+                         params.eta_damping_exp, params.ko_sigma, nthreads)
+    return out
+
+
 def ref_deriv(variant, which, u, h, sz, bflag, beta=None):
     L = ref_lib(variant)
     u = np.ascontiguousarray(u, dtype=np.float64)
@@ -244,38 +272,51 @@ def ghost_fill(u, bl):
     return out
 
 
+# Classic RK4 tableau (Kutta 1901; e.g. Hairer, Norsett & Wanner, Solving
+# ODEs I, table 1.2): stage nodes c = (1/2, 1/2, 1) and weights
+# b = (1/6, 1/3, 1/3, 1/6), written as divisors of dt so that the products
+# round as the library's stage update (dt/2, dt/6, ...) does.
+RK4_NODE_DIV = (2.0, 2.0, 1.0)
+RK4_WEIGHT_DIV = (6.0, 3.0, 3.0, 6.0)
+
+
+def rk4_tableau_step(rhs, u, dt, mask=None, fill=None):
+    """One classic RK4 step of du/dt = rhs(u) for an array u of shape
+    (nfields, npoints).  Only the columns selected by `mask` (all when None)
+    are updated; `fill(v)` (identity when None) refreshes the other columns of
+    every stage argument, as a ghost refresh does, and is applied to u first.
+    The accumulation order is the library's fused stage update:
+    acc = u + dt/6 k1; acc += dt/3 k2; acc += dt/3 k3; u_new = acc + dt/6 k4,
+    and the stage arguments are u + (dt/2) k1, u + (dt/2) k2, u + dt k3."""
+    mk = slice(None) if mask is None else mask
+    fill = fill if fill is not None else (lambda v: v)
+    un = fill(np.array(u, dtype=np.float64, copy=True))
+    k = rhs(un)
+    acc = un.copy()
+    acc[:, mk] = un[:, mk] + (dt / RK4_WEIGHT_DIV[0]) * k[:, mk]
+    for s in range(1, 4):
+        st = un.copy()
+        st[:, mk] = un[:, mk] + (dt / RK4_NODE_DIV[s - 1]) * k[:, mk]
+        k = rhs(fill(st))
+        if s < 3:
+            acc[:, mk] = acc[:, mk] + (dt / RK4_WEIGHT_DIV[s]) * k[:, mk]
+    new = un.copy()
+    new[:, mk] = acc[:, mk] + (dt / RK4_WEIGHT_DIV[3]) * k[:, mk]
+    return new
+
+
 def rk4_step(u, bl, nb, dt, params, nthreads=1):
-    """Classic RK4 step of the oracle RHS with ghost refresh before every
-    stage; only interior points change (textbook tableau).  nb: the side of an
-    nb^3 uniform tiling (lattice-index ghost copy), or None for any block set
-    (oracle_ghost_fill)."""
-    def interior_mask():
-        mk = np.zeros(u.shape[1], dtype=bool)
-        for b in range(len(bl)):
-            n = bl.sz[b][0]
-            v = np.zeros((n, n, n), dtype=bool)
-            v[3:-3, 3:-3, 3:-3] = True
-            mk[bl.offsets[b]:bl.offsets[b] + n ** 3] = v.reshape(-1)
-        return mk
-    mk = interior_mask()
+    """Classic RK4 step (rk4_tableau_step) of the oracle RHS over a block set,
+    with a ghost refresh before every stage; only interior points change.
+    nb: the side of an nb^3 uniform tiling (lattice-index ghost copy), or
+    None for any block set (oracle_ghost_fill)."""
+    mk = np.zeros(u.shape[1], dtype=bool)
+    for b in range(len(bl)):
+        n = bl.sz[b]
+        v = np.zeros((n[2], n[1], n[0]), dtype=bool)
+        v[3:-3, 3:-3, 3:-3] = True
+        mk[bl.offsets[b]:bl.offsets[b] + v.size] = v.reshape(-1)
 
     def fill(v):
         return ghost_fill_uniform(v, bl, nb) if nb is not None else ghost_fill(v, bl)
-    un = fill(u)
-    k1 = rhs_blocks(un, bl, params, nthreads)
-    acc = un.copy(); s = un.copy()
-    acc[:, mk] = un[:, mk] + (dt / 6) * k1[:, mk]
-    s[:, mk] = un[:, mk] + (dt / 2) * k1[:, mk]
-    s = fill(s)
-    k2 = rhs_blocks(s, bl, params, nthreads)
-    acc[:, mk] = acc[:, mk] + (dt / 3) * k2[:, mk]
-    s2 = s.copy(); s2[:, mk] = un[:, mk] + (dt / 2) * k2[:, mk]
-    s2 = fill(s2)
-    k3 = rhs_blocks(s2, bl, params, nthreads)
-    acc[:, mk] = acc[:, mk] + (dt / 3) * k3[:, mk]
-    s3 = s2.copy(); s3[:, mk] = un[:, mk] + dt * k3[:, mk]
-    s3 = fill(s3)
-    k4 = rhs_blocks(s3, bl, params, nthreads)
-    new = un.copy()
-    new[:, mk] = acc[:, mk] + (dt / 6) * k4[:, mk]
-    return new
+    return rk4_tableau_step(lambda v: rhs_blocks(v, bl, params, nthreads), u, dt, mask=mk, fill=fill)

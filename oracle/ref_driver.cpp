@@ -51,6 +51,38 @@ int ref_bssnrhs_sep(double **out, const double **in, unsigned int offset,
     return 0;
 }
 
+// Many blocks on `nthreads` host threads, the way the reference's own GPU
+// example driver runs the CPU path (bssn/cuda_gr/src/gpuBSSNExample.cpp:
+// 267-300: an OpenMP parallel for over blocks, one bssnrhs call each, the
+// parameter globals set once beforehand).  Block b reads in + offsets[b] and
+// writes out + offsets[b]; pmin/pmax/sz hold 3 values per block.
+int ref_bssnrhs_blocks(double **out, const double **in, unsigned int nblocks, const unsigned long long *offsets,
+                       const double *pmin, const double *pmax, const unsigned int *sz, const unsigned int *bflag,
+                       const unsigned int *lambda, const double *lambda_f, double eta_const, double eta_r0,
+                       double eta_damping_exp, double ko_sigma, int nthreads)
+{
+    for (int i = 0; i < 4; i++) bssn::BSSN_LAMBDA[i] = lambda[i];
+    bssn::BSSN_LAMBDA_F[0] = lambda_f[0];
+    bssn::BSSN_LAMBDA_F[1] = lambda_f[1];
+    bssn::ETA_CONST = eta_const;
+    bssn::ETA_R0 = eta_r0;
+    bssn::ETA_DAMPING_EXP = eta_damping_exp;
+    bssn::KO_DISS_SIGMA = ko_sigma;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (unsigned int b = 0; b < nblocks; b++) {
+        double *o[24];
+        const double *u[24];
+        for (int f = 0; f < 24; f++) {
+            o[f] = out[f] + offsets[b];
+            u[f] = in[f] + offsets[b];
+        }
+        const unsigned int off = 0;
+        const unsigned int bf = bflag[b];
+        bssnrhs(o, u, off, pmin + 3 * b, pmax + 3 * b, sz + 3 * b, bf);
+    }
+    return 0;
+}
+
 // Parameters of the gauge variants (bssn/src/parameters.cpp:50-51,123):
 // the eta function R0 * sqrt(igt^ij d_i chi d_j chi) / (1 - chi^p0)^p1
 // (USE_ETA_FUNC) and the Rochester shift weights BSSN_XI (USE_ROCHESTER_GAUGE).

@@ -197,15 +197,32 @@ def ghost_table_stats(bl):
     return {"patches": st[0], "nodes": st[1], "interpolated": st[2]}
 
 
-def _field_ptrs(t):
-    """24 device pointers of the rows of a (24, total) tensor."""
-    if t.dim() != 2 or t.shape[0] != NUM_VARS:
-        raise ValueError("expected a (24, total) tensor")
+def _rows(t, rows, need, device):
+    """Device pointers of the `rows` rows of a (rows, >= need) tensor after
+    checking everything the kernels assume of it: float64, contiguous, on
+    cuda:`device`, each row at least `need` values long (the kernels read and
+    write 8-byte values over the whole block table)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("expected a torch tensor")
+    if t.dim() != 2 or t.shape[0] != rows:
+        raise ValueError(f"expected a ({rows}, total) tensor, got shape {tuple(t.shape)}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"expected float64, got {t.dtype}")
+    if not t.is_cuda or t.device.index != device:
+        raise ValueError(f"tensor must live on cuda:{device}, not {t.device}")
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
+    if t.shape[1] < need:
+        raise ValueError(f"rows hold {t.shape[1]} values, the block table needs {need}")
     base = t.data_ptr()
     stride = t.stride(0) * t.element_size()
-    return (P_D * NUM_VARS)(*[ctypes.cast(base + f * stride, P_D) for f in range(NUM_VARS)])
+    return (P_D * rows)(*[ctypes.cast(base + f * stride, P_D) for f in range(rows)])
+
+
+def _field_ptrs(t, need, device):
+    """24 device pointers of the rows of a checked (24, >= need) tensor."""
+    return _rows(t, NUM_VARS, need, device)
 
 
 class Plan:
@@ -224,23 +241,21 @@ class Plan:
         check(lib().sgr_plan_get_info(self._h, ctypes.byref(i)))
         return i
 
+    def _f(self, t):
+        """Checked field pointers of a (24, total) tensor for this plan."""
+        return _field_ptrs(t, self.blocks.total, self.device)
+
     def rhs(self, out, u, stream=None):
         """Asynchronous RHS on `stream` (default: torch's current stream)."""
-        import torch
-        if u.dtype != torch.float64 or out.dtype != torch.float64 or not u.is_cuda or not out.is_cuda:
-            raise ValueError("u and out must be float64 cuda tensors")
-        if u.shape[1] < self.blocks.total or out.shape[1] < self.blocks.total:
-            raise ValueError("field arrays are shorter than the block table")
-        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
-        check(lib().sgr_rhs(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s)))
+        po, pi = self._f(out), self._f(u)
+        check(lib().sgr_rhs(self._h, po, pi, ctypes.c_void_p(self._stream(u, stream))))
 
     def rhs_events(self, out, u, events, stream=None):
         """sgr_rhs plus hipEvent records around each of the two kernels;
         `events` is a sequence of 4 raw hipEvent_t handles (ints)."""
-        import torch
-        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
+        po, pi = self._f(out), self._f(u)
         ev = (ctypes.c_void_p * 4)(*events)
-        check(lib().sgr_rhs_events(self._h, _field_ptrs(out), _field_ptrs(u), ctypes.c_void_p(s), ev))
+        check(lib().sgr_rhs_events(self._h, po, pi, ctypes.c_void_p(self._stream(u, stream)), ev))
 
     @staticmethod
     def _stream(t, stream):
@@ -250,33 +265,30 @@ class Plan:
     def constraints(self, u, out4, stream=None):
         """Hamiltonian (out4[0]) and momentum (out4[1:4]) constraints at the
         interior points; out4 is a (4, total) float64 tensor."""
-        if out4.dim() != 2 or out4.shape[0] != 4 or not out4.is_contiguous():
-            raise ValueError("out4 must be a contiguous (4, total) tensor")
-        base, st = out4.data_ptr(), out4.stride(0) * out4.element_size()
-        po = (P_D * 4)(*[ctypes.cast(base + c * st, P_D) for c in range(4)])
-        check(lib().sgr_constraints(self._h, _field_ptrs(u), po, ctypes.c_void_p(self._stream(u, stream))))
+        po = _rows(out4, 4, self.blocks.total, self.device)
+        check(lib().sgr_constraints(self._h, self._f(u), po, ctypes.c_void_p(self._stream(u, stream))))
 
     def ghost_fill(self, u, stream=None):
         """Ghost refresh of u in place (copies where grids align, 6-point
         Lagrange interpolation from the owning block elsewhere)."""
-        check(lib().sgr_ghost_fill(self._h, _field_ptrs(u), ctypes.c_void_p(self._stream(u, stream))))
+        check(lib().sgr_ghost_fill(self._h, self._f(u), ctypes.c_void_p(self._stream(u, stream))))
 
     def rk4_stage(self, stage, dt, u_n, u_stage, acc, u_next, stream=None):
-        check(lib().sgr_rk4_stage(self._h, stage, dt, _field_ptrs(u_n), _field_ptrs(u_stage), _field_ptrs(acc),
-                                  _field_ptrs(u_next), ctypes.c_void_p(self._stream(u_n, stream))))
+        check(lib().sgr_rk4_stage(self._h, stage, dt, self._f(u_n), self._f(u_stage), self._f(acc),
+                                  self._f(u_next), ctypes.c_void_p(self._stream(u_n, stream))))
 
     def rk4_step(self, dt, u, s_a, s_b, acc, stream=None):
         """One RK4 step of u in place (ghost refresh + 4 fused stages)."""
-        check(lib().sgr_rk4_step(self._h, dt, _field_ptrs(u), _field_ptrs(s_a), _field_ptrs(s_b),
-                                 _field_ptrs(acc), ctypes.c_void_p(self._stream(u, stream))))
+        check(lib().sgr_rk4_step(self._h, dt, self._f(u), self._f(s_a), self._f(s_b), self._f(acc),
+                                 ctypes.c_void_p(self._stream(u, stream))))
 
     def rk4_graph(self, dt, u, s_a, s_b, acc):
         return Graph(self, dt, u, s_a, s_b, acc)
 
     def check_finite(self, u, stream=None):
-        import torch
-        s = stream if stream is not None else torch.cuda.current_stream(u.device).cuda_stream
-        return lib().sgr_check_finite(self._h, _field_ptrs(u), ctypes.c_void_p(s))
+        """Return code of sgr_check_finite: 0, or SGR_E_NONFINITE (-5) when a
+        padded input value of some block is NaN or infinite."""
+        return lib().sgr_check_finite(self._h, self._f(u), ctypes.c_void_p(self._stream(u, stream)))
 
     def close(self):
         if self._h:
@@ -309,7 +321,9 @@ def rhs_block(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
     """sgr_rhs_block: the reference bssnrhs (bssn/include/rhs.h:33-36) on
     device tensors u, out of shape (24, N), synchronous; reads the padded
     block at `offset`, writes its interior RHS into `out`."""
-    check(lib().sgr_rhs_block(_field_ptrs(out), _field_ptrs(u), offset, (ctypes.c_double * 3)(*pmin),
+    need = offset + sz[0] * sz[1] * sz[2]
+    check(lib().sgr_rhs_block(_field_ptrs(out, need, device), _field_ptrs(u, need, device), offset,
+                              (ctypes.c_double * 3)(*pmin),
                               (ctypes.c_double * 3)(*pmax), (ctypes.c_uint32 * 3)(*sz), bflag,
                               ctypes.byref(prm), device))
     return out
@@ -321,8 +335,8 @@ class Graph:
     def __init__(self, plan, dt, u, s_a, s_b, acc):
         self._g = ctypes.c_void_p()
         self._keep = (plan, u, s_a, s_b, acc)
-        check(lib().sgr_rk4_graph_create(plan._h, dt, _field_ptrs(u), _field_ptrs(s_a), _field_ptrs(s_b),
-                                         _field_ptrs(acc), ctypes.byref(self._g)))
+        check(lib().sgr_rk4_graph_create(plan._h, dt, plan._f(u), plan._f(s_a), plan._f(s_b), plan._f(acc),
+                                         ctypes.byref(self._g)))
 
     def launch(self, stream=None):
         import torch

@@ -19,7 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libsympygr_amd.so")
 ARCH = os.environ.get("SGR_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bssn_kernels.hip", "bssn_fused.hip", "bssn_capi.hip", "bssnrhs_dropin.cpp"]
+SOURCES = ["bssn_kernels.hip", "bssn_capi.hip", "ghost_table.cpp", "bssnrhs_dropin.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",

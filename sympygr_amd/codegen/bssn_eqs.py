@@ -124,15 +124,28 @@ class Stages:
         self.stages.append((name, list(pairs)))
 
 
-def bssn_rhs_stages(contract_beta: bool = True) -> Stages:
+# Fields whose advection the GPU pointwise kernels do not read: the stencil
+# pass folds it, with the Kreiss-Oliger term, into a per-output tail that the
+# kernels add after the algebra (every field).
+TAIL_FIELDS = set(FIELDS)
+
+
+def bssn_rhs_stages(contract_beta: bool = True, tails: bool = False) -> Stages:
     """BSSN RHS, GR/bssn.py:57-104, as staged assignments.  contract_beta:
     the shift's second derivatives enter as igt^{jk} d_j d_k beta^i
     (lap_beta<i>) and d_j (d_k beta^k) (gdivb<j>), formed beside the
-    derivatives, instead of as 18 separate inputs."""
+    derivatives, instead of as 18 separate inputs.  tails: the advection of
+    the TAIL_FIELDS is left out (zero); the caller adds, per output, the tail
+    c_f adv f + sigma KO f (c_f the advection's coefficient in the equation;
+    for B^i: (1 - lambda3) adv Gt^i + lambda2 adv B^i + sigma KO B^i), which
+    makes the outputs equal to the full ones."""
     I = Inputs()
     u = I.u
     P = I.params
-    d1, d2, adv = I.d1, I.d2, I.adv
+    d1, d2 = I.d1, I.d2
+
+    def adv(f):
+        return sp.Integer(0) if tails and f in TAIL_FIELDS else I.adv(f)
     S = Stages()
 
     alpha, chi, K = u["alpha"], u["chi"], u["K"]

@@ -60,6 +60,11 @@ AxisConst axis_const(double pmin, double pmax, uint32_t n)
     return a;
 }
 
+// Workspace bytes per 8^3 tile: RHS pass (staged derivatives in W plus the
+// double tails) and constraint pass (its derivative slots plus KO sums).
+uint64_t rhs_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (RS_NSLOT * wsz + RS_NTAIL * sizeof(double)); }
+uint64_t con_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (NSLOT * wsz + NF * sizeof(double)); }
+
 int validate_params(const sgr_params *p)
 {
     if (!p) return fail(SGR_E_INVALID, "params is null");
@@ -135,280 +140,31 @@ __global__ __launch_bounds__(256) void k_hbm_copy(double2 *__restrict__ dst, con
 
 }  // namespace
 
-struct GhostTable {
-    std::vector<GhostPatch> patches;  // sorted by class
-    uint32_t start[GHOST_CLASSES + 1] = {};
-    std::vector<GhostBox> boxes;      // patches split into kernel work units, sorted by class
-    uint32_t bstart[GHOST_CLASSES + 1] = {};
-    uint64_t nodes = 0, interp_nodes = 0;
-    bool complete = true;
-    std::string msg;
-};
-
-enum class Variant { Fused, Workspace };
-
 struct sgr_plan {
-    Variant variant = Variant::Workspace;
-    int derivs_threads = 512;  // k_derivs workgroup size (512: one point per thread, 256: two)
+    int derivs_threads = 512;  // constraint pass: k_derivs workgroup size (512: one point per thread, 256: two)
     int device = 0;
     sgr_params params{};
     std::vector<BlockDesc> hblocks;
     BlockDesc *d_blocks = nullptr;
     uint32_t *d_tile_block = nullptr;
     uint32_t ntiles = 0;
+    // One device workspace, tile-major, shared by the two passes that use it:
+    //   RHS:         chunk_tiles tiles of RS_NSLOT W values per point, then
+    //                their RS_NTAIL double tails per point
+    //   constraints: cchunk_tiles tiles of NSLOT W values per point, then
+    //                their NF double KO sums per point
     void *d_ws = nullptr;
-    double *d_wsK = nullptr;
     int *d_flag = nullptr;
-    uint64_t np = 0;             // workspace plane stride (points)
-    uint32_t chunk_tiles = 0;    // tiles per launch (workspace capacity)
+    uint32_t chunk_tiles = 0;    // tiles per RHS launch (workspace capacity)
+    uint32_t cchunk_tiles = 0;   // tiles per constraint launch
     uint64_t ws_bytes = 0;
     uint64_t interior = 0;
-    uint32_t nslots = 0;
+    size_t wsz = sizeof(double); // bytes of one staged derivative (W)
     // ghost refresh table
     GhostTable ghost;
     GhostPatch *d_ghost = nullptr;
     GhostBox *d_gbox = nullptr;
 };
-
-// Ghost patch table of a block set (host only).
-//
-// Every block owns the half-open box of its interior nodes widened by half a
-// cell.  Positions are taken to exact integers on the coarsest lattice of unit
-// hmin / (2K), K = 1..64, on which every block origin and every half spacing
-// is an integer, so ownership has no rounding ties.  For each block, every
-// other block whose owned box meets its padded node box contributes one patch:
-// the box of the target's nodes inside the source's owned box, clipped to
-// the target's ghost region that is not beyond a flagged (domain-boundary)
-// face.  The table is complete when these patches cover exactly the target's
-// ghost nodes inside the domain; otherwise the reason is reported (nodes with
-// no owner, overlapping blocks, no common lattice).
-
-static int64_t ceil_div(int64_t a, int64_t b)  // b > 0
-{
-    return a >= 0 ? (a + b - 1) / b : -((-a) / b);
-}
-
-// Splits patch pi (class cls) into boxes for the ghost kernels.  Copy
-// patches: at most GHOST_COPY_NODES nodes a box.  Interpolating patches:
-// pieces n[d] per axis grown greedily until every box keeps the limits of
-// k_ghost_sep (bounded with the largest box and the widest stencil span),
-// each step taking the axis whose split costs least: the per-node loads
-// NX*Ry*Rz + NY*ey*Rz + NZ*ey*ez summed over boxes, plus a fixed cost per box.
-static void split_patch(const GhostPatch &g, uint32_t pi, int cls, std::vector<GhostBox> &out)
-{
-    const int nt[3] = {cls & 1 ? 1 : 6, cls & 2 ? 1 : 6, cls & 4 ? 1 : 6};
-    int64_t n[3] = {1, 1, 1};
-    auto side = [&](const int64_t *k, int d) { return (g.e[d] + k[d] - 1) / k[d]; };
-    auto span = [&](int64_t s, int d) -> int64_t {  // upper bound of ghost_rows for s nodes
-        return nt[d] == 1 ? s : ((s - 1) * g.ht[d] + g.hs[d] - 1) / g.hs[d] + 6;
-    };
-    auto fits = [&](const int64_t *k) {
-        const int64_t s0 = side(k, 0), s1 = side(k, 1), s2 = side(k, 2);
-        if (cls == GHOST_CLASSES - 1) return s0 * s1 * s2 <= GHOST_COPY_NODES;
-        const int64_t Ry = span(s1, 1), Rz = span(s2, 2);
-        return s0 <= GHOST_BOX_MAX && s1 <= GHOST_BOX_MAX && s2 <= GHOST_BOX_MAX && Ry <= GHOST_ROWS_MAX &&
-               Rz <= GHOST_ROWS_MAX && s0 * Ry * Rz + s0 * s1 * Rz <= GHOST_LDS;
-    };
-    auto cost = [&](const int64_t *k) {
-        const int64_t s0 = side(k, 0), s1 = side(k, 1), s2 = side(k, 2);
-        const int64_t Ry = span(s1, 1), Rz = span(s2, 2);
-        const int64_t per = nt[0] * s0 * Ry * Rz + nt[1] * s0 * s1 * Rz + nt[2] * s0 * s1 * s2;
-        return k[0] * k[1] * k[2] * (per + 2048);
-    };
-    while (!fits(n)) {
-        int best = -1;
-        int64_t bc = 0;
-        for (int d = 0; d < 3; d++) {
-            if (side(n, d) <= 1) continue;
-            int64_t m[3] = {n[0], n[1], n[2]};
-            m[d]++;
-            // copies: keep x rows whole while another axis can split
-            const int64_t c = cls == GHOST_CLASSES - 1 ? (d == 0 ? 2 : 1) * (int64_t)1e12 - side(n, d) : cost(m);
-            if (best < 0 || c < bc) best = d, bc = c;
-        }
-        n[best]++;  // all sides 1 always fits, so some side exceeds 1 here
-    }
-    for (int64_t c = 0; c < n[2]; c++)
-        for (int64_t b = 0; b < n[1]; b++)
-            for (int64_t a = 0; a < n[0]; a++) {
-                const int64_t i[3] = {a, b, c};
-                GhostBox x{};
-                x.patch = (int32_t)pi;
-                for (int d = 0; d < 3; d++) {
-                    const int64_t l = g.e[d] * i[d] / n[d], h = g.e[d] * (i[d] + 1) / n[d];
-                    x.lo[d] = g.lo[d] + (int32_t)l;
-                    x.e[d] = (int32_t)(h - l);
-                }
-                if (x.e[0] > 0 && x.e[1] > 0 && x.e[2] > 0) out.push_back(x);
-            }
-}
-
-static GhostTable build_ghost_table(const sgr_block *blocks, uint32_t nb)
-{
-    GhostTable T;
-    auto incomplete = [&](const std::string &m) {
-        if (T.complete) T.complete = false, T.msg = m;
-    };
-    if (nb == 0) return T;
-    struct Lat {
-        int64_t P[3], H[3], m[3], lo[3], hi[3];
-    };
-    std::vector<Lat> L(nb);
-    double O[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hmin = HUGE_VAL;
-    for (uint32_t b = 0; b < nb; b++)
-        for (int d = 0; d < 3; d++) {
-            O[d] = std::fmin(O[d], blocks[b].pmin[d]);
-            hmin = std::fmin(hmin, (blocks[b].pmax[d] - blocks[b].pmin[d]) / (blocks[b].sz[d] - 1));
-        }
-    bool found = false;
-    for (int K = 1; K <= 64 && !found; K++) {
-        const double unit = hmin / (2.0 * K);
-        found = true;
-        for (uint32_t b = 0; b < nb && found; b++)
-            for (int d = 0; d < 3 && found; d++) {
-                const double h = (blocks[b].pmax[d] - blocks[b].pmin[d]) / (blocks[b].sz[d] - 1);
-                const double a = (blocks[b].pmin[d] - O[d]) / unit, c = h / (2.0 * unit);
-                if (std::fabs(a - std::nearbyint(a)) > 1e-6 || std::fabs(c - std::nearbyint(c)) > 1e-6 ||
-                    std::fabs(a) > 1e9 || c > 1e9) {
-                    found = false;
-                    break;
-                }
-                Lat &l = L[b];
-                l.P[d] = (int64_t)std::nearbyint(a);
-                l.H[d] = 2 * (int64_t)std::nearbyint(c);
-                l.m[d] = blocks[b].sz[d];
-                l.lo[d] = l.P[d] + 5 * (l.H[d] / 2);
-                l.hi[d] = l.P[d] + (l.m[d] - 4) * l.H[d] + l.H[d] / 2;
-            }
-    }
-    if (!found) {
-        incomplete("block coordinates share no common lattice (origins and half spacings must be integer "
-                   "multiples of one unit)");
-        return T;
-    }
-    for (uint32_t b = 0; b < nb; b++)
-        for (int d = 0; d < 3; d++)
-            if (L[b].P[d] + L[b].m[d] * L[b].H[d] >= (int64_t(1) << 30)) {
-                incomplete("block set spans more than 2^30 lattice units");
-                return T;
-            }
-    // candidate sources of a target: owned boxes sorted by their low x
-    std::vector<uint32_t> order(nb);
-    for (uint32_t b = 0; b < nb; b++) order[b] = b;
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) { return L[a].lo[0] < L[c].lo[0]; });
-    int64_t wmax = 0;
-    for (uint32_t b = 0; b < nb; b++) wmax = std::max(wmax, L[b].hi[0] - L[b].lo[0]);
-    std::vector<GhostPatch> cls[GHOST_CLASSES];
-    for (uint32_t b = 0; b < nb && T.complete; b++) {
-        const Lat &t = L[b];
-        int64_t al[3], ah[3];
-        uint64_t required = 1, interior = 1, covered = 0;
-        for (int d = 0; d < 3; d++) {
-            al[d] = (blocks[b].bflag >> (2 * d)) & 1 ? 3 : 0;
-            ah[d] = (blocks[b].bflag >> (2 * d + 1)) & 1 ? t.m[d] - 3 : t.m[d];
-            required *= (uint64_t)(ah[d] - al[d]);
-            interior *= (uint64_t)(t.m[d] - 6);
-        }
-        required -= interior;
-        const int64_t x0 = t.P[0], x1 = t.P[0] + (t.m[0] - 1) * t.H[0];
-        auto it = std::lower_bound(order.begin(), order.end(), x0 - wmax,
-                                   [&](uint32_t s, int64_t v) { return L[s].lo[0] < v; });
-        for (; it != order.end() && L[*it].lo[0] <= x1; ++it) {
-            const uint32_t sidx = *it;
-            if (sidx == b) continue;
-            const Lat &s = L[sidx];
-            int64_t lo[3], hi[3];
-            bool empty = false;
-            for (int d = 0; d < 3; d++) {
-                lo[d] = std::max(al[d], ceil_div(s.lo[d] - t.P[d], t.H[d]));
-                hi[d] = std::min(ah[d], ceil_div(s.hi[d] - t.P[d], t.H[d]));
-                empty |= lo[d] >= hi[d];
-            }
-            if (empty) continue;
-            bool meets_interior = true;
-            for (int d = 0; d < 3; d++) meets_interior &= lo[d] < t.m[d] - 3 && hi[d] > 3;
-            if (meets_interior) {
-                incomplete("blocks " + std::to_string(b) + " and " + std::to_string(sidx) + " overlap");
-                break;
-            }
-            GhostPatch g{};
-            g.dst_off = blocks[b].offset;
-            g.src_off = blocks[sidx].offset;
-            g.dn0 = (int32_t)t.m[0];
-            g.dn1 = (int32_t)t.m[1];
-            g.sn0 = (int32_t)s.m[0];
-            g.sn1 = (int32_t)s.m[1];
-            int c = 0;
-            uint64_t vol = 1;
-            for (int d = 0; d < 3; d++) {
-                g.lo[d] = (int32_t)lo[d];
-                g.e[d] = (int32_t)(hi[d] - lo[d]);
-                g.rel[d] = (int32_t)(t.P[d] - s.P[d]);
-                g.ht[d] = (int32_t)t.H[d];
-                g.hs[d] = (int32_t)s.H[d];
-                g.sm[d] = (int32_t)s.m[d];
-                g.inv_hs[d] = 1.0 / (double)s.H[d];
-                if (t.H[d] % s.H[d] == 0 && (t.P[d] - s.P[d]) % s.H[d] == 0) c |= 1 << d;
-                vol *= (uint64_t)g.e[d];
-                // every tap an interior source node (owned box => q in [2, m-4])
-                const int64_t lat0 = g.rel[d] + lo[d] * t.H[d], lat1 = g.rel[d] + (hi[d] - 1) * t.H[d];
-                if (lat0 / s.H[d] < 2 || lat1 / s.H[d] > s.m[d] - 4 || s.m[d] < 12) {
-                    incomplete("ghost patch of block " + std::to_string(b) + " leaves the interior of block " +
-                               std::to_string(sidx));
-                }
-            }
-            covered += vol;
-            T.nodes += vol;
-            if (c != GHOST_CLASSES - 1) T.interp_nodes += vol;
-            cls[c].push_back(g);
-        }
-        if (T.complete && covered != required)
-            incomplete("block " + std::to_string(b) + ": " + std::to_string(covered) + " of " +
-                       std::to_string(required) + " ghost nodes inside the domain have an owning block" +
-                       (covered > required ? " (overlapping blocks)" : " (the rest lie outside every block)"));
-    }
-    if (!T.complete) return T;
-    for (int c = 0; c < GHOST_CLASSES; c++) {
-        T.start[c] = (uint32_t)T.patches.size();
-        T.bstart[c] = (uint32_t)T.boxes.size();
-        for (const GhostPatch &g : cls[c]) {
-            const size_t b0 = T.boxes.size();
-            split_patch(g, (uint32_t)T.patches.size(), c, T.boxes);
-            // every box within the limits k_ghost_sep / k_ghost_copy hold
-            // (exact source rows), and the boxes tile the patch
-            const int nt[3] = {c & 1 ? 1 : 6, c & 2 ? 1 : 6, c & 4 ? 1 : 6};
-            uint64_t vol = 0;
-            for (size_t k = b0; k < T.boxes.size(); k++) {
-                const GhostBox &x = T.boxes[k];
-                const uint64_t v = (uint64_t)x.e[0] * x.e[1] * x.e[2];
-                vol += v;
-                bool ok = true;
-                for (int d = 0; d < 3; d++) ok &= x.lo[d] >= g.lo[d] && x.lo[d] + x.e[d] <= g.lo[d] + g.e[d];
-                if (c == GHOST_CLASSES - 1) {
-                    ok &= v <= (uint64_t)GHOST_COPY_NODES;
-                } else {
-                    const int Ry = ghost_rows(x.lo[1], x.e[1], g.rel[1], g.ht[1], g.hs[1], g.sm[1], nt[1]);
-                    const int Rz = ghost_rows(x.lo[2], x.e[2], g.rel[2], g.ht[2], g.hs[2], g.sm[2], nt[2]);
-                    ok &= x.e[0] <= GHOST_BOX_MAX && x.e[1] <= GHOST_BOX_MAX && x.e[2] <= GHOST_BOX_MAX &&
-                          Ry <= GHOST_ROWS_MAX && Rz <= GHOST_ROWS_MAX &&
-                          (int64_t)x.e[0] * Ry * Rz + (int64_t)x.e[0] * x.e[1] * Rz <= GHOST_LDS;
-                }
-                if (!ok) {
-                    incomplete("internal error: a ghost box exceeds the kernel limits");
-                    return T;
-                }
-            }
-            if (vol != (uint64_t)g.e[0] * g.e[1] * g.e[2]) {
-                incomplete("internal error: ghost boxes do not tile their patch");
-                return T;
-            }
-            T.patches.push_back(g);
-        }
-    }
-    T.start[GHOST_CLASSES] = (uint32_t)T.patches.size();
-    T.bstart[GHOST_CLASSES] = (uint32_t)T.boxes.size();
-    return T;
-}
 
 static void plan_free(sgr_plan *p)
 {
@@ -419,7 +175,6 @@ static void plan_free(sgr_plan *p)
     hipFree(p->d_blocks);
     hipFree(p->d_tile_block);
     hipFree(p->d_ws);
-    hipFree(p->d_wsK);
     hipFree(p->d_flag);
     hipFree(p->d_ghost);
     hipFree(p->d_gbox);
@@ -460,25 +215,13 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     if (!p) return fail(SGR_E_NOMEM, "host allocation failed");
     p->device = device;
     p->params = *params;
-    // Default: the two-pass kernels (LDS-tiled derivative kernel into an HBM
-    // workspace, then two output-group pointwise kernels).  SGR_KERNEL=fused
-    // selects the single-pass kernel (no workspace; slower on gfx950 today
-    // because its register footprint allows one wave per SIMD).
     // SGR_DERIVS_THREADS=256|512 sets the derivative kernel's workgroup size.
     {
-        const char *kv = getenv("SGR_KERNEL");
-        if (kv && !strcmp(kv, "fused")) p->variant = Variant::Fused;
         const char *tv = getenv("SGR_DERIVS_THREADS");
         if (tv && !strcmp(tv, "512")) p->derivs_threads = 512;
         else if (tv && !strcmp(tv, "256")) p->derivs_threads = 256;
     }
-    const bool fusedv = p->variant == Variant::Fused;
-    if (fusedv && (params->gauge != SGR_GAUGE_STANDARD || params->eta_mode != SGR_ETA_CONST)) {
-        delete p;
-        return fail(SGR_E_UNSUPPORTED, "the fused kernel evaluates the standard gauge with constant eta only");
-    }
-    const int tdim[3] = {fusedv ? FUSED_TX : T, fusedv ? FUSED_TY : T, fusedv ? FUSED_TZ : T};
-    p->nslots = fusedv ? 0 : NSLOT;
+    const int tdim[3] = {T, T, T};
     std::vector<uint32_t> tile_block;
     uint64_t ws_base = 0;
     for (uint32_t b = 0; b < nblocks; b++) {
@@ -505,9 +248,43 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
             return fail(SGR_E_INVALID, "too many tiles");
         }
         for (uint64_t t = 0; t < nt; t++) tile_block.push_back(b);
-        if (!fusedv) ws_base += nt * TILE_PTS;
+        ws_base += nt * TILE_PTS;
         p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
         p->hblocks.push_back(d);
+    }
+    p->ntiles = (uint32_t)tile_block.size();
+    // The workspace holds at most `chunk_tiles` tiles of the RHS pass (budget
+    // SGR_WS_GIB, default 40 GiB); larger plans run as several launches over
+    // equal tile ranges.  The constraint pass stages more per tile and runs
+    // in as many launches as that same allocation needs (at least one tile).
+    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
+    p->wsz = wsz;
+    const uint64_t rhs_tile = rhs_tile_bytes(wsz), con_tile = con_tile_bytes(wsz);
+    {
+        double gib = 40.0;
+        const char *gv = getenv("SGR_WS_GIB");
+        if (gv) gib = atof(gv);
+        uint64_t cap = (uint64_t)(gib * 1073741824.0) / rhs_tile;
+        if (cap < 1) cap = 1;
+        const uint64_t nt = tile_block.size();
+        // equal chunks (no short last launch)
+        const uint64_t nchunks = nt ? (nt + cap - 1) / cap : 1;
+        p->chunk_tiles = (uint32_t)((nt + nchunks - 1) / nchunks);
+        p->ws_bytes = nt ? std::max<uint64_t>(p->chunk_tiles * rhs_tile, con_tile) : 0;
+        p->cchunk_tiles = (uint32_t)std::min<uint64_t>(nt, p->ws_bytes / con_tile);
+    }
+    // optional cap on the plan's device memory (SGR_DEVICE_MEM_LIMIT_GIB):
+    // a plan that would need more is refused before anything is allocated
+    {
+        const char *lv = getenv("SGR_DEVICE_MEM_LIMIT_GIB");
+        const uint64_t need = p->ws_bytes + nblocks * sizeof(BlockDesc) + tile_block.size() * sizeof(uint32_t);
+        if (lv && (double)need > atof(lv) * 1073741824.0) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "plan needs %.3f GiB of device memory, over SGR_DEVICE_MEM_LIMIT_GIB=%s",
+                     need / 1073741824.0, lv);
+            delete p;
+            return fail(SGR_E_NOMEM, msg);
+        }
     }
     int ndev = 0;
     hipError_t derr = hipGetDeviceCount(&ndev);
@@ -515,29 +292,6 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         delete p;
         return derr != hipSuccess ? hip_fail(derr, "hipGetDeviceCount") : fail(SGR_E_INVALID, "device index out of range");
     }
-    p->ntiles = (uint32_t)tile_block.size();
-    // The workspace holds at most `chunk_tiles` tiles (budget SGR_WS_GIB,
-    // default 40 GiB); larger plans run as several launches over equal tile
-    // ranges.  Plane stride: the chunk's point count padded by a few KiB so
-    // consecutive slot planes do not share one power-of-two alignment.
-    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
-    {
-        double gib = 40.0;
-        const char *gv = getenv("SGR_WS_GIB");
-        if (gv) gib = atof(gv);
-        const uint64_t per_tile = (uint64_t)TILE_PTS * ((uint64_t)p->nslots * wsz + NF * sizeof(double));
-        uint64_t cap = (uint64_t)(gib * 1073741824.0) / (per_tile ? per_tile : 1);
-        if (cap < 1) cap = 1;
-        const uint64_t nt = tile_block.size();
-        // equal chunks (no short last launch)
-        const uint64_t nchunks = nt ? (nt + cap - 1) / cap : 1;
-        p->chunk_tiles = (uint32_t)(fusedv ? nt : (nt + nchunks - 1) / nchunks);
-        uint64_t pad = 2048 + 64;
-        const char *pv = getenv("SGR_WS_PAD");
-        if (pv) pad = strtoull(pv, nullptr, 10);
-        p->np = fusedv ? 0 : (uint64_t)p->chunk_tiles * TILE_PTS + (nt ? pad : 0);
-    }
-    p->ws_bytes = fusedv ? 0 : p->np * (uint64_t)p->nslots * wsz + p->np * (uint64_t)NF * sizeof(double);
 
     int cur = 0;
     hipGetDevice(&cur);
@@ -549,31 +303,37 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
                                           : hip_fail(err, what);
     };
     if (e != hipSuccess) return bail(e, "hipSetDevice");
+    // SGR_FAULT_ALLOC=k: the k-th device allocation of this call reports
+    // out-of-memory without allocating (exercises the clean-up of a
+    // partially built plan)
+    int alloc_no = 0;
+    const char *fv = getenv("SGR_FAULT_ALLOC");
+    const int fault_at = fv ? atoi(fv) : 0;
+    auto dev_alloc = [&](void **ptr, size_t bytes) -> hipError_t {
+        if (++alloc_no == fault_at) return hipErrorOutOfMemory;
+        return hipMalloc(ptr, bytes);
+    };
     if (nblocks) {
-        if ((e = hipMalloc(&p->d_blocks, nblocks * sizeof(BlockDesc))) != hipSuccess) return bail(e, "hipMalloc(blocks)");
+        if ((e = dev_alloc((void **)&p->d_blocks, nblocks * sizeof(BlockDesc))) != hipSuccess)
+            return bail(e, "hipMalloc(blocks)");
         if ((e = hipMemcpy(p->d_blocks, p->hblocks.data(), nblocks * sizeof(BlockDesc), hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(blocks)");
-        if ((e = hipMalloc(&p->d_tile_block, tile_block.size() * sizeof(uint32_t))) != hipSuccess)
+        if ((e = dev_alloc((void **)&p->d_tile_block, tile_block.size() * sizeof(uint32_t))) != hipSuccess)
             return bail(e, "hipMalloc(tiles)");
         if ((e = hipMemcpy(p->d_tile_block, tile_block.data(), tile_block.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(tiles)");
-        if (!fusedv) {
-            if ((e = hipMalloc(&p->d_ws, p->np * (uint64_t)p->nslots * wsz)) != hipSuccess)
-                return bail(e, "hipMalloc(workspace)");
-            if ((e = hipMalloc(&p->d_wsK, p->np * (uint64_t)NF * sizeof(double))) != hipSuccess)
-                return bail(e, "hipMalloc(ko workspace)");
-        }
+        if ((e = dev_alloc(&p->d_ws, p->ws_bytes)) != hipSuccess) return bail(e, "hipMalloc(workspace)");
     }
-    if ((e = hipMalloc(&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
+    if ((e = dev_alloc((void **)&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
     p->ghost = build_ghost_table(blocks, nblocks);
     if (!p->ghost.patches.empty()) {
         const size_t gb = p->ghost.patches.size() * sizeof(GhostPatch);
-        if ((e = hipMalloc(&p->d_ghost, gb)) != hipSuccess) return bail(e, "hipMalloc(ghost table)");
+        if ((e = dev_alloc((void **)&p->d_ghost, gb)) != hipSuccess) return bail(e, "hipMalloc(ghost table)");
         if ((e = hipMemcpy(p->d_ghost, p->ghost.patches.data(), gb, hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(ghost table)");
         const size_t bb = p->ghost.boxes.size() * sizeof(GhostBox);
-        if ((e = hipMalloc(&p->d_gbox, bb)) != hipSuccess) return bail(e, "hipMalloc(ghost boxes)");
+        if ((e = dev_alloc((void **)&p->d_gbox, bb)) != hipSuccess) return bail(e, "hipMalloc(ghost boxes)");
         if ((e = hipMemcpy(p->d_gbox, p->ghost.boxes.data(), bb, hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(ghost boxes)");
     }
@@ -595,7 +355,7 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
     info->tiles = plan->ntiles;
     info->interior_points = plan->interior;
     info->workspace_bytes = plan->ws_bytes;
-    info->derivative_slots = plan->nslots;
+    info->derivative_slots = RS_NSLOT + RS_NTAIL;
     info->chunks = plan->chunk_tiles ? (plan->ntiles + plan->chunk_tiles - 1) / plan->chunk_tiles : 0;
     return SGR_OK;
 }
@@ -614,29 +374,19 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.tile_block = plan->d_tile_block;
     a.ntiles = plan->ntiles;
     a.ws = plan->d_ws;
-    a.wsK = plan->d_wsK;
-    a.np = plan->np;
+    a.wsT = (double *)((char *)plan->d_ws + (uint64_t)plan->chunk_tiles * TILE_PTS * RS_NSLOT * plan->wsz);
     a.params = device_params(plan->params);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
     a.derivs_threads = plan->derivs_threads;
     for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
     a.rk.stage = -1;
-    if (rk) {
-        if (plan->variant == Variant::Fused)
-            return fail(SGR_E_UNSUPPORTED, "RK4 stages need the two-pass kernels (unset SGR_KERNEL=fused)");
-        a.rk = *rk;
-    }
+    if (rk) a.rk = *rk;
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
     hipError_t e = hipSuccess;
-    if (plan->variant == Variant::Fused) {
-        e = launch_fused(a, stream);
-        // one kernel: the second event pair brackets nothing
-        if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
-        if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
-    } else if (plan->ntiles <= plan->chunk_tiles) {
+    if (plan->ntiles <= plan->chunk_tiles) {
         e = launch_rhs(a, stream);
     } else {
         // several workspace chunks: events[0..1] bracket the whole call,
@@ -672,8 +422,6 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
 int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream)
 {
     if (!plan || !in || !out) return fail(SGR_E_INVALID, "null argument");
-    if (plan->variant == Variant::Fused)
-        return fail(SGR_E_UNSUPPORTED, "constraints need the two-pass kernels (unset SGR_KERNEL=fused)");
     LaunchArgs a{};
     OutPtrs co{};
     for (int f = 0; f < NF; f++) {
@@ -688,8 +436,7 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     a.tile_block = plan->d_tile_block;
     a.ntiles = plan->ntiles;
     a.ws = plan->d_ws;
-    a.wsK = plan->d_wsK;
-    a.np = plan->np;
+    a.wsK = (double *)((char *)plan->d_ws + (uint64_t)plan->cchunk_tiles * TILE_PTS * NSLOT * plan->wsz);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
     a.derivs_threads = plan->derivs_threads;
@@ -697,9 +444,9 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
     hipError_t e = hipSuccess;
-    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->chunk_tiles) {
+    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->cchunk_tiles) {
         a.tile0 = t0;
-        a.ntiles = plan->ntiles - t0 < plan->chunk_tiles ? plan->ntiles - t0 : plan->chunk_tiles;
+        a.ntiles = plan->ntiles - t0 < plan->cchunk_tiles ? plan->ntiles - t0 : plan->cchunk_tiles;
         e = launch_constraints(a, co, stream);
     }
     if (cur != plan->device) hipSetDevice(cur);
@@ -884,50 +631,80 @@ int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipSt
 }  // extern "C"
 
 // Per-block entry points (sgr_rhs_block, sgr_bssnrhs_host).  A Dendro driver
-// calls bssnrhs block after block, so the plan and the staging buffers are
-// kept between calls in one process-wide context, as the reference keeps its
-// one memory pool: a one-block plan, rebuilt only when the device, the block
-// size or the kernel-selecting parameters (advection, precision, SGR_KERNEL)
-// change; the block's offset, coordinates and bflag are re-uploaded per call.
-// Calls are serialised by a mutex.  The buffers are held until the process
-// ends.
+// calls bssnrhs block after block, often from several host threads at once
+// (an OpenMP loop over blocks), so every calling thread keeps its own staging
+// context: a one-block plan, a stream of its own and its device and pinned
+// host buffers.  The plan is rebuilt only when the device, the block size or
+// the kernel-selecting parameters (advection, precision) change; the block's
+// offset, coordinates and bflag are uploaded with each call.  Contexts of
+// different threads run concurrently on the GPU; each is held until its
+// thread ends.
 namespace {
 struct BlockStage {
-    std::mutex m;
     sgr_plan *plan = nullptr;
     int device = -1;
     uint32_t sz[3] = {0, 0, 0};
     int32_t adv = -1, prec = -1;
-    uint64_t cap = 0;           // points per field held by d_in, d_out, h_out
+    hipStream_t stream = nullptr;
+    uint64_t cap = 0;            // points per field held by the buffers
     double *d_in = nullptr, *d_out = nullptr;
-    double *h_out = nullptr;    // pinned
+    double *h_in = nullptr, *h_out = nullptr;  // pinned
+    BlockDesc *h_desc = nullptr;                // pinned copy of the block descriptor
+    void release()
+    {
+        if (device >= 0) hipSetDevice(device);
+        if (stream) hipStreamSynchronize(stream);
+        if (plan) sgr_plan_destroy(plan);
+        hipFree(d_in);
+        hipFree(d_out);
+        hipHostFree(h_in);
+        hipHostFree(h_out);
+        hipHostFree(h_desc);
+        if (stream) hipStreamDestroy(stream);
+        plan = nullptr;
+        d_in = d_out = h_in = h_out = nullptr;
+        h_desc = nullptr;
+        stream = nullptr;
+        cap = 0;
+        device = -1;
+    }
+    ~BlockStage() { release(); }
 };
-BlockStage g_stage;
+thread_local BlockStage g_stage;
 
-// The plan of the staging context set up for block b (sz, offset, geometry)
-// and params on `device`; the caller holds g_stage.m and the device is current.
+// The context's plan set up for block b (sz, offset, geometry) and params
+// on `device`, and its stream; the device is current.
 int stage_plan(const sgr_block &b, const sgr_params *params, int device)
 {
     BlockStage &S = g_stage;
-    const char *kv = getenv("SGR_KERNEL");
-    const Variant want = kv && !strcmp(kv, "fused") ? Variant::Fused : Variant::Workspace;
-    const bool same = S.plan && S.device == device && S.sz[0] == b.sz[0] && S.sz[1] == b.sz[1] &&
-                      S.sz[2] == b.sz[2] && S.adv == params->adv_mode && S.prec == params->precision &&
-                      S.plan->variant == want;
+    if (S.device != device) S.release();
+    if (!S.stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        S.device = device;
+    }
+    if (!S.h_desc) {
+        const hipError_t e = hipHostMalloc((void **)&S.h_desc, sizeof(BlockDesc), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(block)");
+    }
+    const bool same = S.plan && S.sz[0] == b.sz[0] && S.sz[1] == b.sz[1] && S.sz[2] == b.sz[2] &&
+                      S.adv == params->adv_mode && S.prec == params->precision;
     if (!same) {
-        if (S.plan) sgr_plan_destroy(S.plan);
+        if (S.plan) {
+            hipStreamSynchronize(S.stream);
+            sgr_plan_destroy(S.plan);
+        }
         S.plan = nullptr;
         const int rc = sgr_plan_create(&S.plan, &b, 1, params, device);
         if (rc) return rc;
-        S.device = device;
         for (int d = 0; d < 3; d++) S.sz[d] = b.sz[d];
         S.adv = params->adv_mode;
         S.prec = params->precision;
         return SGR_OK;
     }
-    // same size: new offset, coordinates, bflag and physics parameters
-    if (S.plan->variant == Variant::Fused && (params->gauge != SGR_GAUGE_STANDARD || params->eta_mode != SGR_ETA_CONST))
-        return fail(SGR_E_UNSUPPORTED, "the fused kernel evaluates the standard gauge with constant eta only");
+    // same size: new offset, coordinates, bflag and physics parameters; the
+    // descriptor goes up on the context's stream, ahead of the kernels
+    HIP_TRY(hipStreamSynchronize(S.stream), "hipStreamSynchronize");  // the pinned copy is free again
     BlockDesc &d = S.plan->hblocks[0];
     d.offset = b.offset;
     d.bflag = b.bflag;
@@ -936,23 +713,28 @@ int stage_plan(const sgr_block &b, const sgr_params *params, int device)
         d.ax[k] = axis_const(b.pmin[k], b.pmax[k], b.sz[k]);
     }
     S.plan->params = *params;
-    HIP_TRY(hipMemcpy(S.plan->d_blocks, &d, sizeof(BlockDesc), hipMemcpyHostToDevice), "hipMemcpy(block)");
+    *S.h_desc = d;
+    HIP_TRY(hipMemcpyAsync(S.plan->d_blocks, S.h_desc, sizeof(BlockDesc), hipMemcpyHostToDevice, S.stream),
+            "hipMemcpyAsync(block)");
     return SGR_OK;
 }
 
-int stage_buffers(uint64_t n, int device)
+int stage_buffers(uint64_t n)
 {
     BlockStage &S = g_stage;
-    if (S.d_in && S.cap >= n && S.device == device) return SGR_OK;
+    if (S.d_in && S.cap >= n) return SGR_OK;
     hipFree(S.d_in);
     hipFree(S.d_out);
+    hipHostFree(S.h_in);
     hipHostFree(S.h_out);
-    S.d_in = S.d_out = S.h_out = nullptr;
+    S.d_in = S.d_out = S.h_in = S.h_out = nullptr;
     S.cap = 0;
     hipError_t e = hipMalloc(&S.d_in, n * NF * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&S.d_out, n * NF * sizeof(double));
-    if (e == hipSuccess) e = hipHostMalloc(&S.h_out, n * NF * sizeof(double), hipHostMallocDefault);
-    if (e != hipSuccess) return hip_fail(e, "staging buffers");
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S.h_in, n * NF * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&S.h_out, n * NF * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? fail(SGR_E_NOMEM, "staging buffers: out of memory")
+                                                         : hip_fail(e, "staging buffers");
     S.cap = n;
     return SGR_OK;
 }
@@ -984,14 +766,13 @@ int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NU
     sgr_block b;
     int rc = block_checks(pmin, pmax, sz, bflag, params, offset, b);
     if (rc) return rc;
-    std::lock_guard<std::mutex> lock(g_stage.m);
     int cur = 0;
     hipGetDevice(&cur);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     rc = stage_plan(b, params, device);
-    if (rc == SGR_OK) rc = sgr_rhs(g_stage.plan, out, in, nullptr);
+    if (rc == SGR_OK) rc = sgr_rhs(g_stage.plan, out, in, g_stage.stream);
     if (rc == SGR_OK) {
-        hipError_t e = hipStreamSynchronize(nullptr);
+        hipError_t e = hipStreamSynchronize(g_stage.stream);
         if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
     }
     hipSetDevice(cur);
@@ -1009,27 +790,31 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
     int rc = block_checks(pmin, pmax, sz, bflag, params, 0, b);  // staged at offset 0
     if (rc) return rc;
     const uint64_t n = (uint64_t)sz[0] * sz[1] * sz[2];
-    std::lock_guard<std::mutex> lock(g_stage.m);
     int cur = 0;
     hipGetDevice(&cur);
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
-    rc = stage_buffers(n, device);
-    if (rc == SGR_OK) rc = stage_plan(b, params, device);
+    rc = stage_plan(b, params, device);  // leaves the context's stream idle
+    if (rc == SGR_OK) rc = stage_buffers(n);
+    BlockStage &S = g_stage;
     hipError_t e = hipSuccess;
-    for (int f = 0; f < NF && rc == SGR_OK && e == hipSuccess; f++)
-        e = hipMemcpy(g_stage.d_in + f * n, in[f] + offset, n * sizeof(double), hipMemcpyHostToDevice);
-    if (rc == SGR_OK && e != hipSuccess) rc = hip_fail(e, "staging input");
+    if (rc == SGR_OK) {
+        // gather the 24 padded input arrays into pinned memory, one H2D
+        for (int f = 0; f < NF; f++) std::memcpy(S.h_in + f * n, in[f] + offset, n * sizeof(double));
+        e = hipMemcpyAsync(S.d_in, S.h_in, n * NF * sizeof(double), hipMemcpyHostToDevice, S.stream);
+        if (e != hipSuccess) rc = hip_fail(e, "staging input");
+    }
     if (rc == SGR_OK) {
         double *o[NF];
         const double *i[NF];
         for (int f = 0; f < NF; f++) {
-            o[f] = g_stage.d_out + f * n;
-            i[f] = g_stage.d_in + f * n;
+            o[f] = S.d_out + f * n;
+            i[f] = S.d_in + f * n;
         }
-        rc = sgr_rhs(g_stage.plan, o, i, nullptr);
+        rc = sgr_rhs(S.plan, o, i, S.stream);
     }
     if (rc == SGR_OK) {
-        e = hipMemcpy(g_stage.h_out, g_stage.d_out, n * NF * sizeof(double), hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(S.h_out, S.d_out, n * NF * sizeof(double), hipMemcpyDeviceToHost, S.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(S.stream);
         if (e != hipSuccess) rc = hip_fail(e, "copying RHS back");
     }
     hipSetDevice(cur);
@@ -1040,7 +825,7 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
         for (uint32_t k = 3; k < nz - 3; k++)
             for (uint32_t j = 3; j < ny - 3; j++) {
                 const uint64_t row = (uint64_t)nx * (j + (uint64_t)ny * k);
-                std::memcpy(out[f] + offset + row + 3, g_stage.h_out + f * n + row + 3, (nx - 6) * sizeof(double));
+                std::memcpy(out[f] + offset + row + 3, S.h_out + f * n + row + 3, (nx - 6) * sizeof(double));
             }
     return SGR_OK;
 }

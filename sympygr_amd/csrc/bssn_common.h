@@ -53,21 +53,41 @@ __host__ __device__ constexpr int sym_slot(int j, int k)
     return j > k ? sym_slot(k, j) : (j == 0 ? k : (j == 1 ? 2 + k : 5));
 }
 
-// Workspace layout.  Tile-major (1, default): slot s of point p of tile t at
-// ws[t*TILE_PTS*nslots + s*TILE_PTS + p], a tile's slots one contiguous run
-// (slot stride TILE_PTS).  Slot-major (0): ws[s*SS + t*TILE_PTS + p], SS the
-// plane stride the host passes.  The host allocation covers both.  On the
-// 512 x 32^3 RHS tile-major ran at 1065-1074 Mpts/s in five runs, slot-major
-// at 1041 and 1099 (its plane alignment varies with the allocation); groups
-// of 4-128 tiles per slot run were no faster than single tiles.
-#ifndef SGR_WS_TILE_MAJOR
-#define SGR_WS_TILE_MAJOR 1
-#endif
+// Derivative workspace of the RHS pass (k_stencil -> k_pointwise).  The
+// stencil pass stages what the pointwise algebra reads at every interior
+// point that is not on a flagged face (it stores the outputs of face points
+// itself):
+//   [0,45)    first derivative of field f along d, 3 q(f) + d, for
+//             alpha chi K Gt0..2 beta0..2 (q = f) and gt0..5 (q = f - 3)
+//             (B and At enter the algebra without derivatives)
+//   [45,57)   second derivatives of alpha (45 + s) and chi (51 + s)
+//   [57,66)   igt^{lm} d_l d_m f of the shift (57 + f - 6) and gt (60 + f - 12)
+//   [66,69)   d_j (d_k beta^k), 66 + j
+// and, always in double, the tail of each of the 24 outputs: the terms
+// linear in the field's own advective and Kreiss-Oliger stencils, added
+// after the algebra, c_f beta.ad f + sigma KO f, with c_f the advection's
+// coefficient in the equation (lambda0 for alpha, lambda1 or, in the
+// Rochester gauge, xi1 for the shift, 1 otherwise) and for B^i
+// (1 - lambda3) beta.ad Gt^i + lambda2 beta.ad B^i + sigma KO B^i
+// (sigma KO B^i alone in the Rochester gauge, where B is not evolved).
+constexpr int RS_NSLOT = 69;
+constexpr int RS_NTAIL = NF;
+__host__ __device__ constexpr bool rs_has_d1(int f) { return f <= 8 || (f >= 12 && f <= 17); }
+__host__ __device__ constexpr int rs_d1(int f, int d) { return 3 * (f <= 8 ? f : f - 3) + d; }
+__host__ __device__ constexpr int rs_d2(int f, int s) { return (f == 1 ? 51 : 45) + s; }
+__host__ __device__ constexpr int rs_lap(int f) { return f <= 8 ? 57 + (f - 6) : 60 + (f - 12); }
+__host__ __device__ constexpr int rs_gdb(int j) { return 66 + j; }
+__host__ __device__ constexpr int rs_tail(int f) { return f; }
+
+// Workspace layout, tile-major: slot s of point p (x fastest in the 8^3
+// tile) of tile t at ws[t * TILE_PTS * nslots + s * TILE_PTS + p], so a
+// tile's slots are one contiguous run and every wave reads and writes 512
+// contiguous bytes per slot.
 __host__ __device__ constexpr uint64_t ws_tile_base(uint64_t tile, int nslots)
 {
-    return tile * (uint64_t)TILE_PTS * (SGR_WS_TILE_MAJOR ? (uint64_t)nslots : 1u);
+    return tile * (uint64_t)TILE_PTS * (uint64_t)nslots;
 }
-__host__ __device__ constexpr uint64_t ws_slot_stride(uint64_t ss) { return SGR_WS_TILE_MAJOR ? TILE_PTS : ss; }
+constexpr uint64_t WS_SS = TILE_PTS;  // slot stride
 
 // The derivative workspace is streamed: written once by the derivative pass,
 // read once per pointwise kernel, far larger than L2.  Non-temporal accesses

@@ -41,6 +41,7 @@ namespace sgr {
 struct TileCoord {
     int lt;           // tile id within the block
     int i0, j0, k0;   // padded coordinate of the tile's first interior point
+    int oi, oj, ok;   // first coordinate the tile owns along each axis
 };
 
 __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t tile)
@@ -50,8 +51,13 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
     const int tx = c.lt % B.ntile[0];
     const int ty = (c.lt / B.ntile[0]) % B.ntile[1];
     const int tz = c.lt / (B.ntile[0] * B.ntile[1]);
-    // last tile shifted back to end at the interior's end (overlap is benign:
-    // every tile owns its own workspace points and writes identical results)
+    // The last tile along an axis is shifted back to end at the interior's
+    // end, so it overlaps its neighbour.  Every tile evaluates derivatives at
+    // all its points (each tile has its own workspace), but an output point
+    // belongs to exactly one tile: the one whose unshifted range
+    // [GW + T t, GW + T (t + 1)) holds it.  Stores skip the points a shifted
+    // tile shares with the previous one, so read-modify-write outputs (the RK4
+    // accumulator) are updated once.
     auto origin = [](int tt, int n) {
         const int o = GW + T * tt, last = n - GW - T;
         return o < last ? o : (last > GW ? last : GW);
@@ -59,6 +65,9 @@ __device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t til
     c.i0 = origin(tx, B.n[0]);
     c.j0 = origin(ty, B.n[1]);
     c.k0 = origin(tz, B.n[2]);
+    c.oi = GW + T * tx;
+    c.oj = GW + T * ty;
+    c.ok = GW + T * tz;
     return c;
 }
 
@@ -75,7 +84,7 @@ __device__ __forceinline__ int lds_halo(int e) { return e + 10 * (e / TH); }
 template <typename W, bool UPWIND, int NT>
 __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                 const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
-                                                double *__restrict__ wsK, uint64_t SS, bool all_d1, uint32_t tile0)
+                                                double *__restrict__ wsK, bool all_d1, uint32_t tile0)
 {
     constexpr int NHALO = TH * TH * TH;            // 2744 halo elements
     // LDS rows of the halo tile are RS doubles apart: 48 dwords, so the rows
@@ -115,10 +124,10 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     double bv[PPT][3];  // shift at the point
     double igp[PPT][6];  // inverse conformal metric at the point, for the contracted fields
     // this tile's workspace: its 512 points contiguous within every slot
-    // (layout: ws_tile_base, ws_slot_stride)
+    // (layout: ws_tile_base, slot stride WS_SS)
     W *__restrict__ wsb = ws + ws_tile_base(ct, NSLOT);
     double *__restrict__ kb = wsK + ws_tile_base(ct, NF);
-    SS = ws_slot_stride(SS);
+    constexpr uint64_t SS = WS_SS;
 #pragma unroll
     for (int p = 0; p < PPT; p++) {
         pk[p] = tc.k0 + lz + ZS * p;
@@ -367,6 +376,310 @@ __device__ __forceinline__ void rk_store(const RkArgs &rk, int f, uint64_t pp, d
     out_put(&rk.next.p[f][pp], u0 + rk.cn * k);
 }
 
+// eta of the Gamma-driver at one point (bssn/src/rhs.cpp:206-219): the
+// constant ETA_CONST, damped by (ETA_R0 / r)^ETA_DAMPING_EXP beyond ETA_R0;
+// or, under eta_func (GR/bssn.py:52), BSSN_ETA_R0 sqrt(igt^ij d_i chi d_j chi)
+// / (1 - chi^p0)^p1 from the point's gt, d chi and chi.
+__device__ __forceinline__ double eta_const_at(const Params &P, double x, double y, double z)
+{
+    double e = P.eta_const;
+    const double r = radius(x, y, z);
+    if (r >= P.eta_r0) e *= pow(P.eta_r0 / r, P.eta_damping_exp);
+    return e;
+}
+__device__ __forceinline__ double eta_func_at(const Params &P, const double g[6], const double dchi[3], double chi)
+{
+    double ig[6], h[6];
+    inverse_metric(g, ig);
+    h[0] = dchi[0] * dchi[0];
+    h[1] = dchi[0] * dchi[1];
+    h[2] = dchi[0] * dchi[2];
+    h[3] = dchi[1] * dchi[1];
+    h[4] = dchi[1] * dchi[2];
+    h[5] = dchi[2] * dchi[2];
+    return P.eta_func_r0 * sqrt(metric_contract(ig, h)) * pow(1.0 - pow(chi, P.eta_power[0]), -P.eta_power[1]);
+}
+
+__device__ __forceinline__ double tail_add(double v, double tail)
+{
+    SGR_EXACT
+    return v + tail;
+}
+
+// First, advective, Kreiss-Oliger and (want2) pure second derivative of one
+// field along one axis at one point.  A tile that reaches a flagged face on
+// this axis (lo / hi, uniform over the tile) takes the per-point closure
+// forms from LDS; every other tile loads the 7 taps u[-3..3] once into
+// registers and evaluates the interior forms from them (same expressions,
+// same order: bit-equal to the LDS forms).
+template <bool UPWIND>
+__device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, int n, bool lo, bool hi, double beta,
+                                              const AxisConst &a, bool want2, double &g, double &q, double &k,
+                                              double &h)
+{
+    if (lo || hi) {
+        g = d1(c, s, pos, n, lo, hi, a);
+        q = UPWIND ? adv(c, s, pos, n, lo, hi, beta, a) : g;
+        k = ko(c, s, pos, n, lo, hi, a);
+        h = want2 ? d2(c, s, pos, n, lo, hi, a) : 0.0;
+    } else {
+        double t[7];
+#pragma unroll
+        for (int o = 0; o < 7; o++) t[o] = c[(o - 3) * s];
+        g = d1_int(t + 3, 1, a);
+        q = UPWIND ? ad_int(t + 3, 1, beta > 0.0, a) : g;
+        k = ko_int(t + 3, 1, a);
+        h = want2 ? d2_int(t + 3, 1, a) : 0.0;
+    }
+}
+
+// k_stencil: one 512-thread workgroup per 8^3 tile, one point per thread.
+// Field by field, the 14^3 halo tile is staged in LDS (the next field's tile
+// in flight from HBM meanwhile) and every thread evaluates its point's
+// stencils: first, advective and Kreiss-Oliger along each axis; for alpha,
+// chi, the shift and gt also the second derivatives, the mixed ones as
+// compositions D_y D_x, D_z D_x, D_z D_y through the LDS-staged inner
+// derivatives (bssn/scripts/bssnrhs_derivs.h:7-9).  It stages for the
+// pointwise kernels what the algebra reads (the RS_* slots) and each
+// output's tail (its advection and Kreiss-Oliger terms, in double); on a
+// flagged-face point the tail is the whole output, the Sommerfeld value
+// (bssn/src/rhs.cpp:833-977) plus sigma KO.
+template <typename W, bool UPWIND>
+__global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDesc *__restrict__ blocks,
+                                                     const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
+                                                     double *__restrict__ wsT, Params P, uint32_t tile0)
+{
+    constexpr int NT = 512;
+    constexpr int NHALO = TH * TH * TH;            // 2744 halo elements
+    // LDS rows of the halo tile are RS doubles apart: 48 dwords, so the rows
+    // a wave reads in one LDS pass (8 x-values each) fall on disjoint banks
+    constexpr int RS = 24, PS = RS * TH;
+    constexpr int NPF = (NHALO + NT - 1) / NT;     // prefetch registers per thread
+    __shared__ double U[PS * TH];        // x + RS (y + 14 z), halo coordinates
+    __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
+    __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
+    __shared__ double GD[3 * TILE_PTS];  // d_j (d_k beta^k) accumulated over the shift fields
+
+    const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
+    const uint32_t tile = tile0 + ct;
+    const BlockDesc &B = blocks[tile_block[tile]];
+    const TileCoord tc = tile_coord(B, tile);
+    const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+    const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
+    const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
+    const AxisConst a0 = B.ax[0], a1 = B.ax[1], a2 = B.ax[2];
+    // face closures apply to the three points next to a flagged face: only
+    // tiles that reach them take the per-point closure branches (uniform)
+    const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + T - 1 >= n0 - GW - 3;
+    const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + T - 1 >= n1 - GW - 3;
+    const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + T - 1 >= n2 - GW - 3;
+
+    const int t = threadIdx.x;
+    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    const int pi = tc.i0 + lx, pj = tc.j0 + ly, pk = tc.k0 + lz;
+    const bool valid = pi < n0 - GW && pj < n1 - GW && pk < n2 - GW;
+    const bool face = valid && ((lo0 && pi == GW) || (hi0 && pi == n0 - GW - 1) || (lo1 && pj == GW) ||
+                                (hi1 && pj == n1 - GW - 1) || (lo2 && pk == GW) || (hi2 && pk == n2 - GW - 1));
+    const uint64_t wp = (uint64_t)t;  // the point's position in the tile (x fastest)
+    const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk);
+    const double sigma = P.ko_sigma;
+    // advection coefficient of each equation's own field (the tails)
+    const double c_alpha = (double)P.lambda[0];
+    const double c_beta = P.rochester ? (double)P.xi[1] : (double)P.lambda[1];
+    double bv[3];     // shift at the point
+    double igp[6];    // inverse conformal metric at the point (identity outside the interior)
+    {
+        double g[6];
+#pragma unroll
+        for (int d = 0; d < 3; d++) bv[d] = valid ? in.p[6 + d][gp] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; q++) g[q] = valid ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
+        inverse_metric(g, igp);
+    }
+    W *__restrict__ wsb = ws + ws_tile_base(ct, RS_NSLOT);
+    double *__restrict__ tb = wsT + ws_tile_base(ct, RS_NTAIL);
+    constexpr uint64_t SS = WS_SS;
+
+    // halo element -> offset in the block (or -1 outside the padded block)
+    int32_t src[NPF];
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+        const int e = t + NT * q;
+        const int ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
+        const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
+        src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
+    }
+    {
+        const double *__restrict__ s0 = in.p[0] + B.offset;
+#pragma unroll
+        for (int q = 0; q < NPF; q++)
+            if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int f = 0; f < NF; f++) {
+        // fetch the next field's halo tile into registers; it lands in LDS
+        // after this field's stencils, so the HBM latency overlaps them
+        double pf[NPF];
+        if (f + 1 < NF) {
+            const double *__restrict__ sn = in.p[f + 1] + B.offset;
+#pragma unroll
+            for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
+        }
+        const bool has_d2 = d2q(f) >= 0;  // alpha, chi, the shift, gt (uniform)
+        const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + GW))];
+        const int cx = lx + T * ((ly + GW) + TH * (lz + GW)), cy = lx + T * (ly + T * (lz + GW));
+        double h0 = 0.0, h1 = 0.0, h2 = 0.0;  // pure second derivatives xx, yy, zz
+        if (!valid) {
+            if (has_d2) {  // a tile point outside the interior: the stages' own conditions
+                DX[cx] = pi < n0 - GW ? d1(c, 1, pi, n0, clo0, chi0, a0) : 0.0;
+                DY[cy] = pi < n0 - GW && pj < n1 - GW ? d1(c, RS, pj, n1, clo1, chi1, a1) : 0.0;
+            }
+        } else {
+            double g0, g1, g2, q0, q1, q2, k0, k1, k2;
+            axis_stencils<UPWIND>(c, 1, pi, n0, clo0, chi0, bv[0], a0, has_d2, g0, q0, k0, h0);
+            axis_stencils<UPWIND>(c, RS, pj, n1, clo1, chi1, bv[1], a1, has_d2, g1, q1, k1, h1);
+            axis_stencils<UPWIND>(c, PS, pk, n2, clo2, chi2, bv[2], a2, has_d2, g2, q2, k2, h2);
+            if (has_d2) {  // the tile points' own cells of the mixed-term stages
+                DX[cx] = g0;
+                DY[cy] = g1;
+            }
+            double advf, kof;
+            {
+                SGR_EXACT
+                advf = (bv[0] * q0 + bv[1] * q1) + bv[2] * q2;
+                kof = (k0 + k1) + k2;
+            }
+            if (rs_has_d1(f)) {
+                ws_put(&wsb[rs_d1(f, 0) * SS + wp], (W)g0);
+                ws_put(&wsb[rs_d1(f, 1) * SS + wp], (W)g1);
+                ws_put(&wsb[rs_d1(f, 2) * SS + wp], (W)g2);
+            }
+            if (f <= 1) {  // pure second derivatives of alpha and chi
+                ws_put(&wsb[rs_d2(f, 0) * SS + wp], (W)h0);
+                ws_put(&wsb[rs_d2(f, 3) * SS + wp], (W)h1);
+                ws_put(&wsb[rs_d2(f, 5) * SS + wp], (W)h2);
+            }
+            if (face) {
+                // the Sommerfeld value replaces the right-hand side, then + sigma KO
+                const double x = coord(B.pmin[0], pi, a0.h), y = coord(B.pmin[1], pj, a1.h);
+                const double z = coord(B.pmin[2], pk, a2.h);
+                ws_put(&tb[rs_tail(f) * SS + wp],
+                       ko_add(sommerfeld(x, y, z, 1.0 / radius(x, y, z), g0, g1, g2, c[0], bc_falloff(f), bc_asym(f)),
+                              sigma, kof));
+            } else if (f >= 3 && f <= 5) {
+                // Gt^i: its tail, and the Gt part of B^i's tail (completed at B^i)
+                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(advf, sigma, kof));
+                {
+                    SGR_EXACT
+                    ws_put(&tb[rs_tail(f + 6) * SS + wp], (1.0 - (double)P.lambda[3]) * advf);
+                }
+            } else if (f >= 9 && f <= 11) {
+                double v = 0.0;  // Rochester gauge: B is not evolved
+                if (!P.rochester) {
+                    SGR_EXACT
+                    v = ws_get(&tb[rs_tail(f) * SS + wp]) + (double)P.lambda[2] * advf;
+                }
+                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
+            } else {
+                const double cf = f == 0 ? c_alpha : (f >= 6 && f <= 8) ? c_beta : 1.0;
+                double v;
+                {
+                    SGR_EXACT
+                    v = cf * advf;
+                }
+                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
+            }
+        }
+
+        if (has_d2) {  // uniform over the workgroup
+            // inner derivatives of the mixed terms, valid wherever the block
+            // has them.  D_x u only where D_y and D_z read it and the tile
+            // points did not write it: the y halo rows of the tile's z planes
+            // (NA), and the tile's y rows of the z halo planes (NB); the
+            // y-halo x z-halo corners are never read.  D_y u on the z halo
+            // planes.
+            constexpr int NA = T * (2 * GW) * T, NB = T * T * (2 * GW);
+            for (int e = t; e < NA + NB; e += NT) {
+                int xx, yy, zz;
+                if (e < NA) {
+                    const int ry = (e / T) % (2 * GW);
+                    xx = e % T;
+                    yy = ry < GW ? ry : ry + T;
+                    zz = GW + e / (T * 2 * GW);
+                } else {
+                    const int r = e - NA, rz = r / (T * T);
+                    xx = r % T;
+                    yy = GW + (r / T) % T;
+                    zz = rz < GW ? rz : rz + T;
+                }
+                const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
+                double v = 0.0;
+                if (gi < n0 - GW && gj < n1 && gk < n2)
+                    v = d1(&U[(xx + GW) + RS * (yy + TH * zz)], 1, gi, n0, clo0, chi0, a0);
+                DX[xx + T * (yy + TH * zz)] = v;
+            }
+            for (int e = t; e < T * T * (2 * GW); e += NT) {
+                const int rz = e / (T * T);
+                const int xx = e % T, yy = (e / T) % T, zz = rz < GW ? rz : rz + T;
+                const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
+                double v = 0.0;
+                if (gi < n0 - GW && gj < n1 - GW && gk < n2)
+                    v = d1(&U[(xx + GW) + RS * ((yy + GW) + TH * zz)], RS, gj, n1, clo1, chi1, a1);
+                DY[xx + T * (yy + T * zz)] = v;
+            }
+            __syncthreads();
+            if (valid) {
+                const double *cxp = &DX[lx + T * ((ly + GW) + TH * (lz + GW))];
+                const double *cyp = &DY[lx + T * (ly + T * (lz + GW))];
+                const double m01 = d1(cxp, T, pj, n1, clo1, chi1, a1);       // D_y D_x
+                const double m02 = d1(cxp, T * TH, pk, n2, clo2, chi2, a2);  // D_z D_x
+                const double m12 = d1(cyp, T * T, pk, n2, clo2, chi2, a2);   // D_z D_y
+                if (f == 0 || f == 1) {
+                    ws_put(&wsb[rs_d2(f, 1) * SS + wp], (W)m01);
+                    ws_put(&wsb[rs_d2(f, 2) * SS + wp], (W)m02);
+                    ws_put(&wsb[rs_d2(f, 4) * SS + wp], (W)m12);
+                } else {
+                    // the shift and gt_ij enter only contracted: igt^{lm} d_l d_m f
+                    // is stored, and for the shift d_j d_k beta^k is accumulated
+                    SGR_EXACT
+                    const double *ig = igp;
+                    const double L = ((((ig[0] * h0 + 2.0 * (ig[1] * m01)) + 2.0 * (ig[2] * m02)) + ig[3] * h1) +
+                                      2.0 * (ig[4] * m12)) + ig[5] * h2;
+                    ws_put(&wsb[rs_lap(f) * SS + wp], (W)L);
+                    if (f <= 8) {  // shift beta^k, k = f - 6: d_j d_k beta^k for j = 0, 1, 2
+                        const double r0 = f == 6 ? h0 : (f == 7 ? m01 : m02);
+                        const double r1 = f == 6 ? m01 : (f == 7 ? h1 : m12);
+                        const double r2 = f == 6 ? m02 : (f == 7 ? m12 : h2);
+                        double *gd = &GD[wp];
+                        if (f == 6) {
+                            gd[0] = r0;
+                            gd[TILE_PTS] = r1;
+                            gd[2 * TILE_PTS] = r2;
+                        } else if (f == 7) {
+                            gd[0] = gd[0] + r0;
+                            gd[TILE_PTS] = gd[TILE_PTS] + r1;
+                            gd[2 * TILE_PTS] = gd[2 * TILE_PTS] + r2;
+                        } else {
+                            ws_put(&wsb[rs_gdb(0) * SS + wp], (W)(gd[0] + r0));
+                            ws_put(&wsb[rs_gdb(1) * SS + wp], (W)(gd[TILE_PTS] + r1));
+                            ws_put(&wsb[rs_gdb(2) * SS + wp], (W)(gd[2 * TILE_PTS] + r2));
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();  // every read of U (and DX, DY) for field f is done
+        if (f + 1 < NF) {
+#pragma unroll
+            for (int q = 0; q < NPF; q++)
+                if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = pf[q];
+        }
+        __syncthreads();
+    }
+}
+
 #define SGR_PW_NAME k_pointwise_g1
 #define SGR_PW_ALG "gen/bssn_algebra_g1.inc"
 #include "pointwise_body.h"
@@ -385,19 +698,15 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     W *ws = (W *)a.ws;
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
-    if (a.derivs_threads == 512)
-        hipExtLaunchKernelGGL((k_derivs<W, UPWIND, 512>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                              a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
-    else
-        hipExtLaunchKernelGGL((k_derivs<W, UPWIND, 256>), dim3(a.ntiles), dim3(256), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                              a.blocks, a.tile_block, ws, a.wsK, a.np, false, a.tile0);
+    hipExtLaunchKernelGGL((k_stencil<W, UPWIND>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
+                          a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const dim3 g(2 * a.ntiles), b(256);
-    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, UPWIND, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
-    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, UPWIND, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsK, a.np, a.params, a.rk, a.tile0);
+    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
+    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
+                          a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
     return hipGetLastError();
 }
 
@@ -419,7 +728,7 @@ hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s)
 template <typename W, bool UPWIND>
 __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, const W *__restrict__ ws,
-                                                     uint64_t SS, uint32_t tile0)
+                                                     uint32_t tile0)
 {
     const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t tile = tile0 + (wg >> 1);
@@ -429,9 +738,10 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
     const int i = tc.i0 + (l & 7), j = tc.j0 + ((l >> 3) & 7), k = tc.k0 + (l >> 6);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
+    if (i < tc.oi || j < tc.oj || k < tc.ok) return;  // owned by the previous tile
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
     const W *__restrict__ wsb = ws + ws_tile_base(wg >> 1, NSLOT) + l;
-    SS = ws_slot_stride(SS);
+    constexpr uint64_t SS = WS_SS;
     typedef double R;
 #define BSSN_U(F_) ((R)in.p[F_][pp])
 #define BSSN_D1(F_, D_) ((R)ws_get(&wsb[slot_d1(F_, D_) * SS]))
@@ -455,14 +765,14 @@ static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStr
     W *ws = (W *)a.ws;
     if (a.derivs_threads == 512)
         hipLaunchKernelGGL((k_derivs<W, UPWIND, 512>), dim3(a.ntiles), dim3(512), 0, s, a.in, a.blocks, a.tile_block, ws,
-                           a.wsK, a.np, true, a.tile0);
+                           a.wsK, true, a.tile0);
     else
         hipLaunchKernelGGL((k_derivs<W, UPWIND, 256>), dim3(a.ntiles), dim3(256), 0, s, a.in, a.blocks, a.tile_block, ws,
-                           a.wsK, a.np, true, a.tile0);
+                           a.wsK, true, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_constraints<W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
-                       a.tile_block, (const W *)ws, a.np, a.tile0);
+                       a.tile_block, (const W *)ws, a.tile0);
     return hipGetLastError();
 }
 

@@ -1,6 +1,9 @@
 // C++ drop-in bssnrhs(...) over the C ABI (see include/bssn_dropin.h).
 #include <atomic>
-#include <stdexcept>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/bssn_dropin.h"
@@ -29,8 +32,24 @@ std::atomic<int> g_device{0};
 std::atomic<int32_t> g_gauge{SGR_GAUGE_STANDARD};
 std::atomic<int32_t> g_eta{SGR_ETA_CONST};
 
-void run(double **uzipVarsRHS, const double **uZipVars, unsigned int offset, const double *ptmin,
-         const double *ptmax, const unsigned int *sz, unsigned int bflag, const char *who)
+// first failure since the last sgr_dropin_status call
+std::mutex g_status_m;
+int g_status = SGR_OK;
+std::string g_status_msg;
+std::atomic<bool> g_reported{false};
+
+void record_error(int rc, const std::string &msg)
+{
+    std::lock_guard<std::mutex> lock(g_status_m);
+    if (g_status == SGR_OK) {
+        g_status = rc;
+        g_status_msg = msg;
+    }
+    if (!g_reported.exchange(true)) std::fprintf(stderr, "sympygr_amd: %s\n", msg.c_str());
+}
+
+// The parameters a bssnrhs call uses: the bssn:: globals and the switches.
+sgr_params current_params()
 {
     sgr_params p;
     for (int i = 0; i < 4; i++) p.lambda[i] = bssn::BSSN_LAMBDA[i];
@@ -48,15 +67,54 @@ void run(double **uzipVarsRHS, const double **uZipVars, unsigned int offset, con
     p.eta_func_r0 = bssn::BSSN_ETA_R0;
     p.eta_power[0] = bssn::BSSN_ETA_POWER[0];
     p.eta_power[1] = bssn::BSSN_ETA_POWER[1];
+    return p;
+}
+
+void run(double **uzipVarsRHS, const double **uZipVars, unsigned int offset, const double *ptmin,
+         const double *ptmax, const unsigned int *sz, unsigned int bflag, const char *who)
+{
+    const sgr_params p = current_params();
     const uint32_t s[3] = {sz[0], sz[1], sz[2]};
     const int rc = sgr_bssnrhs_host(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, s, bflag, &p, g_device);
     if (rc != SGR_OK) {
+        // bssnrhs has no error channel and may run inside a parallel loop, so
+        // nothing is thrown: the block's interior RHS becomes NaN (which the
+        // caller's own checks see, as they would a diverging evolution) and
+        // the first error is kept for sgr_dropin_status
         char buf[512];
         sgr_last_error(buf, sizeof buf);
-        throw std::runtime_error(std::string(who) + ": " + buf);
+        record_error(rc, std::string(who) + ": " + buf);
+        if (uzipVarsRHS && sz && sz[0] > 6 && sz[1] > 6 && sz[2] > 6) {
+            for (int f = 0; f < SGR_NUM_VARS; f++) {
+                if (!uzipVarsRHS[f]) continue;
+                for (unsigned int k = 3; k < sz[2] - 3; k++)
+                    for (unsigned int j = 3; j < sz[1] - 3; j++)
+                        for (unsigned int i = 3; i < sz[0] - 3; i++)
+                            uzipVarsRHS[f][offset + i + sz[0] * (j + sz[1] * k)] = NAN;
+            }
+        }
     }
 }
 }  // namespace
+
+extern "C" void sgr_dropin_params(void *p)
+{
+    if (p) *(sgr_params *)p = current_params();
+}
+
+extern "C" int sgr_dropin_status(char *buf, size_t len)
+{
+    std::lock_guard<std::mutex> lock(g_status_m);
+    const int rc = g_status;
+    if (buf && len) {
+        const size_t m = g_status_msg.size() < len - 1 ? g_status_msg.size() : len - 1;
+        std::memcpy(buf, g_status_msg.data(), m);
+        buf[m] = 0;
+    }
+    g_status = SGR_OK;
+    g_status_msg.clear();
+    return rc;
+}
 
 extern "C" void sgr_dropin_configure(int32_t adv_mode, int32_t precision, int device)
 {

@@ -154,6 +154,14 @@ __device__ __forceinline__ double adv(const P *u, int s, int pos, int n, bool lo
 
 // ---------------------------------------------------------------- Kreiss-Oliger
 template <typename P>
+__device__ __forceinline__ double ko_int(const P *u, int s, const AxisConst &a)
+{
+    SGR_EXACT
+    return a.ko_pre * (-tap(u, s, -3) + 6.0 * tap(u, s, -2) - 15.0 * tap(u, s, -1) + 20.0 * tap(u, s, 0) -
+                       15.0 * tap(u, s, 1) + 6.0 * tap(u, s, 2) - tap(u, s, 3));
+}
+
+template <typename P>
 __device__ __forceinline__ double ko(const P *u, int s, int pos, int n, bool lo, bool hi, const AxisConst &a)
 {
     SGR_EXACT
@@ -176,8 +184,7 @@ __device__ __forceinline__ double ko(const P *u, int s, int pos, int n, bool lo,
                     3.0 * tap(u, s, 1)) / a.smr2;
         return (tap(u, s, -3) - 3.0 * tap(u, s, -2) + 3.0 * tap(u, s, -1) - tap(u, s, 0)) / a.smr3;
     }
-    return a.ko_pre * (-tap(u, s, -3) + 6.0 * tap(u, s, -2) - 15.0 * tap(u, s, -1) + 20.0 * tap(u, s, 0) -
-                       15.0 * tap(u, s, 1) + 6.0 * tap(u, s, 2) - tap(u, s, 3));
+    return ko_int(u, s, a);
 }
 
 }  // namespace sgr

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import sympygr_amd as sgr
@@ -115,12 +116,9 @@ def test_rejects_bad_params():
     assert _plan_error(_one(), p).code == -1
 
 
-def test_gauge_defaults_and_fused_kernel_restriction(monkeypatch):
+def test_gauge_defaults():
     p = sgr.params()
     assert (p.gauge, p.eta_mode, list(p.xi), p.eta_func_r0, list(p.eta_power)) == (0, 0, [0, 0, 0], 1.31, [2.0, 2.0])
-    monkeypatch.setenv("SGR_KERNEL", "fused")
-    e = _plan_error(_one(), sgr.params(gauge="rochester"))
-    assert e.code == -2 and "standard gauge" in str(e)
 
 
 def test_null_arguments_are_errors_not_crashes():
@@ -139,3 +137,94 @@ def test_no_gpu_is_reported_as_hip_error():
         pytest.skip("GPU present")
     e = _plan_error(_one())
     assert e.code == -3
+
+
+def test_tensor_arguments_are_checked_before_any_kernel_sees_them():
+    """Every plan entry point takes its field pointers through one checker:
+    wrong shape, dtype, device or layout is a ValueError on the host, never a
+    pointer handed to a kernel."""
+    import torch
+    from sympygr_amd import _field_ptrs, _rows
+    with pytest.raises(ValueError, match=r"\(24, total\)"):
+        _field_ptrs(torch.zeros((23, 10), dtype=torch.float64), 10, 0)
+    with pytest.raises(ValueError, match="float64"):
+        _field_ptrs(torch.zeros((24, 10), dtype=torch.float32), 10, 0)
+    with pytest.raises(ValueError, match="cuda:0"):
+        _field_ptrs(torch.zeros((24, 10), dtype=torch.float64), 10, 0)
+    with pytest.raises(ValueError, match=r"\(4, total\)"):
+        _rows(torch.zeros((24, 10), dtype=torch.float64), 4, 10, 0)
+    with pytest.raises(ValueError, match="torch tensor"):
+        _field_ptrs(np.zeros((24, 10)), 10, 0)
+
+
+def test_device_memory_limit_refuses_a_plan_before_allocating(monkeypatch):
+    """SGR_DEVICE_MEM_LIMIT_GIB caps what one plan may take: a plan that
+    needs more is refused with SGR_E_NOMEM before any device call (so this
+    runs without a GPU), naming its need and the limit."""
+    bl = BlockList()
+    bl.add((38, 38, 38), (-1.0,) * 3, (1.0,) * 3, 0)
+    monkeypatch.setenv("SGR_DEVICE_MEM_LIMIT_GIB", "0.0001")
+    e = _plan_error(bl)
+    assert e.code == -4 and "SGR_DEVICE_MEM_LIMIT_GIB" in str(e) and "GiB" in str(e)
+
+
+def test_application_definitions_of_bssn_globals_override_the_library(tmp_path):
+    """A program that defines the bssn:: parameter globals (as Dendro's
+    parameters.cpp does) and links libsympygr_amd.so: the drop-in reads the
+    program's values, not the library's weak defaults."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "dropin_link_check")
+    libdir = os.path.dirname(sgr.LIB_PATH)
+    r = subprocess.run(["g++", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                        "-I", os.path.join(root, "include"), os.path.join(root, "tests", "dropin_link_check.cpp"),
+                        "-L", libdir, "-lsympygr_amd", f"-Wl,-rpath,{libdir}", "-L/opt/rocm/lib",
+                        "-Wl,-rpath,/opt/rocm/lib", "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    v = out.stdout.split()
+    assert [int(x) for x in v[:4]] == [1, 0, 1, 0]
+    assert [float(x) for x in v[4:13]] == [0.75, 0.25, 1.25, 30.0, 2.0, 0.37, 1.5, 1.0, 3.0]
+    assert [int(x) for x in v[13:]] == [2, 1, 3]
+
+
+def test_dropin_failure_leaves_nan_and_a_status_instead_of_throwing():
+    """bssnrhs cannot report errors (void, reference signature), and may run
+    inside an OpenMP loop: a failed call (here: no GPU, or an invalid block)
+    writes NaN into the block's interior RHS, leaves its ghosts untouched and
+    keeps the first failure for sgr_dropin_status, which clears it."""
+    import torch
+    L = ctypes.CDLL(sgr.LIB_PATH)
+    fn = getattr(L, "_Z7bssnrhsPPdPPKdRKjS2_S2_PS4_S5_")
+    PD = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.POINTER(PD), ctypes.POINTER(PD), ctypes.POINTER(ctypes.c_uint), PD, PD,
+                   ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]
+    fn.restype = None
+    L.sgr_dropin_status.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.sgr_dropin_status(None, 0)  # clear
+    n = 13
+    u = np.ones((24, n ** 3))
+    out = np.full_like(u, 3.0)
+    pi = (PD * 24)(*[u[f].ctypes.data_as(PD) for f in range(24)])
+    po_ = (PD * 24)(*[out[f].ctypes.data_as(PD) for f in range(24)])
+    # bflag 64 is invalid everywhere, GPU or not
+    fn(po_, pi, ctypes.byref(ctypes.c_uint(0)), (ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1),
+       (ctypes.c_uint * 3)(n, n, n), ctypes.byref(ctypes.c_uint(64)))
+    v = out.reshape(24, n, n, n)
+    assert np.isnan(v[:, 3:-3, 3:-3, 3:-3]).all()
+    ghost = np.ones((n, n, n), dtype=bool)
+    ghost[3:-3, 3:-3, 3:-3] = False
+    assert (v[:, ghost] == 3.0).all()
+    buf = ctypes.create_string_buffer(512)
+    assert L.sgr_dropin_status(buf, 512) == -1 and b"bflag" in buf.value
+    assert L.sgr_dropin_status(buf, 512) == 0 and buf.value == b""
+    if not torch.cuda.is_available():
+        out[:] = 3.0
+        fn(po_, pi, ctypes.byref(ctypes.c_uint(0)), (ctypes.c_double * 3)(0, 0, 0), (ctypes.c_double * 3)(1, 1, 1),
+           (ctypes.c_uint * 3)(n, n, n), ctypes.byref(ctypes.c_uint(0)))
+        assert np.isnan(v[:, 3:-3, 3:-3, 3:-3]).all()
+        assert L.sgr_dropin_status(buf, 512) == -3

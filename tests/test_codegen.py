@@ -14,9 +14,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_committed_algebra_is_up_to_date():
     E = gen.generate(write=False)
     text = gen.HEADER.format(flops=E.flops, ntemps=E.ntemps, ninputs=len(E.inputs)) + E.text
-    for rel in ("sympygr_amd/csrc/gen/bssn_algebra.inc", "oracle/gen/bssn_algebra.inc"):
+    with open(os.path.join(ROOT, "oracle/gen/bssn_algebra.inc")) as f:
+        assert f.read() == text, "oracle/gen/bssn_algebra.inc is stale: run python -m sympygr_amd.codegen"
+    for g, (_, gtext) in gen.generate_groups(write=False).items():
+        rel = f"sympygr_amd/csrc/gen/bssn_algebra_{g}.inc"
         with open(os.path.join(ROOT, rel)) as f:
-            assert f.read() == text, f"{rel} is stale: run python -m sympygr_amd.codegen"
+            assert f.read() == gtext, f"{rel} is stale: run python -m sympygr_amd.codegen"
 
 
 def test_input_set_matches_reference_derivative_lists():
@@ -85,3 +88,47 @@ def test_factorised_ricci_equals_textbook_contraction():
         q = sum(C2s(k, l, i) * C1u(j, k, l) + C2s(k, l, j) * C1u(i, k, l) for k in E3 for l in E3)
         q += sum(C2s(k, i, m) * C1u(k, j, m) for k in E3 for m in E3)
         assert sp.simplify(q - direct[(i, j)]) == 0
+
+
+def test_tail_variant_differs_from_full_algebra_by_the_tails():
+    """The GPU kernels run the algebra with every advection left out and add
+    per-output tails.  Symbolically, full - tails equals exactly the tail's
+    advection part: c_f adv_f (c = lambda0 for alpha, lambda1 for the shift,
+    1 otherwise), and (1 - lambda3) adv_Gt^i + lambda2 adv_B^i for B^i; in the
+    Rochester alternatives xi1 adv_beta^i for the shift and 0 for B."""
+    from sympygr_amd.codegen.bssn_eqs import RHS_NAMES, Inputs
+    full, tl = bssn_rhs_stages(), bssn_rhs_stages(tails=True)
+    fs = dict(p for _, pairs in full.stages for p in pairs)
+    ts = dict(p for _, pairs in tl.stages for p in pairs)
+    I = Inputs()
+    P = I.params
+    adv = I.adv
+    # stages differ only where an advection enters (the Gt equation)
+    for s, e in fs.items():
+        d = sp.expand(e - ts[s])
+        if s.name.startswith("Gt_rhs_s"):
+            assert d == sp.expand(adv(f"Gt{s.name[-1]}")), s
+        else:
+            assert d == 0, s
+    for k, ((n, ef), (_, et)) in enumerate(zip(full.outputs, tl.outputs)):
+        f = FIELDS[k]
+        d = ef - et
+        if f.startswith("B"):
+            i = int(f[1])
+            # both read the stage symbol Gt_rhs_s<i>, which in the full
+            # algebra carries + adv Gt^i (checked above)
+            want = P["lambda2"] * adv(f) - P["lambda3"] * adv(f"Gt{i}")
+            assert sp.expand(d - want) == 0, (n, d)  # + adv Gt^i arrives through the stage
+        elif f.startswith("Gt"):
+            assert d == 0, n  # its advection arrives through the Gt_rhs_s stage (checked above)
+        elif f == "alpha":
+            assert sp.expand(d - P["lambda0"] * adv(f)) == 0, n
+        elif f.startswith("beta"):
+            assert sp.expand(d - P["lambda1"] * adv(f)) == 0, n
+        else:
+            assert sp.expand(d - adv(f)) == 0, (n, d)
+    for k, e in full.alt_outputs.items():
+        d = sp.expand(e - tl.alt_outputs[k])
+        f = FIELDS[k]
+        assert d == (P["xi1"] * adv(f) if f.startswith("beta") else 0), f
+    assert len(RHS_NAMES) == 24

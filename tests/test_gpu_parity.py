@@ -20,13 +20,8 @@ def _torch():
     return torch
 
 
-def _run_gpu(u, sz, pmin, pmax, bflag, adv, precision="fp64", variant=None, **pkw):
+def _run_gpu(u, sz, pmin, pmax, bflag, adv, precision="fp64", **pkw):
     torch = _torch()
-    import os
-    if variant is not None:
-        os.environ["SGR_KERNEL"] = variant
-    else:
-        os.environ.pop("SGR_KERNEL", None)
     import sympygr_amd as sgr
     from sympygr_amd.data import BlockList
     bl = BlockList()
@@ -251,9 +246,8 @@ def test_cpp_dropin_bssnrhs_sep_and_gauge_switch():
         L.sgr_dropin_configure(0, 0, 0)
 
 
-@pytest.mark.parametrize("variant", ["fused", "workspace"])
 @pytest.mark.parametrize("n_int", [7, 9, 12, 17, 24])
-def test_uneven_block_sizes_both_kernels(variant, n_int):
+def test_uneven_block_sizes(n_int):
     """Interior sizes that are not multiples of the tile (last tile shifted
     back, or narrower than a tile) against the oracle, all faces flagged and
     none."""
@@ -263,25 +257,24 @@ def test_uneven_block_sizes_both_kernels(variant, n_int):
     pmin, pmax = (-0.7, -0.4, -0.9), (-0.7 + 0.07 * (n - 1), -0.4 + 0.06 * (n - 1), -0.9 + 0.08 * (n - 1))
     u = data.perturbed_flat_block(seed=5, n=n, amp=1e-2, pmin=pmin, pmax=pmax)
     for bflag in (0, 63):
-        print(f"{variant} n_int={n_int} bflag={bflag}", flush=True)
-        out = _run_gpu(u, (n, n, n), pmin, pmax, bflag, "upwind", variant=variant)
+        print(f"n_int={n_int} bflag={bflag}", flush=True)
+        out = _run_gpu(u, (n, n, n), pmin, pmax, bflag, "upwind")
         ref = po.rhs_block(u, (n, n, n), pmin, pmax, bflag, po.make_params(upwind=True))
         err = normwise(interior(out, (n, n, n)), interior(ref, (n, n, n)))
-        assert err.max() <= FP64_TOL, (variant, n_int, bflag, err)
+        assert err.max() <= FP64_TOL, (n_int, bflag, err)
 
 
-@pytest.mark.parametrize("variant", ["fused", "workspace"])
-def test_fixtures_both_kernels_mixed_and_fp64(variant):
+def test_fixtures_mixed_and_fp64():
     import pyoracle as po
     for name in ("c0_upwind_b63", "c0_centred_b63", "fake_upwind_b42"):
         u, sz, pmin, pmax, c = case_input(name)
         adv = "upwind" if c["variant"] == "u" else "centred"
-        out = _run_gpu(u, sz, pmin, pmax, c["bflag"], adv, variant=variant)
-        assert normwise(interior(out, sz), golden(name)).max() <= FP64_TOL, (variant, name)
-        outm = _run_gpu(u, sz, pmin, pmax, c["bflag"], adv, precision="mixed", variant=variant)
+        out = _run_gpu(u, sz, pmin, pmax, c["bflag"], adv)
+        assert normwise(interior(out, sz), golden(name)).max() <= FP64_TOL, name
+        outm = _run_gpu(u, sz, pmin, pmax, c["bflag"], adv, precision="mixed")
         ref = po.rhs_block(u, sz, pmin, pmax, c["bflag"], po.make_params(upwind=adv == "upwind"))
         em = normwise(interior(outm, sz), interior(ref, sz)).max()
-        print(variant, name, f"mixed err {em:.2e}", flush=True)
+        print(name, f"mixed err {em:.2e}", flush=True)
         # 1e-6 on the near-flat data; the analytic stress data (fields up to
         # 1e4, strong cancellation in the Ricci terms) is held to 4e-6
         assert em <= (1e-6 if name.startswith("c0") else 4e-6)

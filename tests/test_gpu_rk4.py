@@ -10,10 +10,10 @@ NB, N = 2, 16
 DT = 0.25 * (2.0 / (NB * N))
 
 
-def _setup():
+def _setup(n=N):
     import torch
     from sympygr_amd import data
-    bl = data.uniform_tiling(nb=NB, n=N, lo=-1.0, hi=1.0, stagger=0.5, faces=True)
+    bl = data.uniform_tiling(nb=NB, n=n, lo=-1.0, hi=1.0, stagger=0.5, faces=True)
     u = data.fill_blocks(bl, kind="perturbed", seed=3, amp=1e-2, lo=-1.0, L=2.0)
     return torch, bl, u
 
@@ -21,10 +21,10 @@ def _setup():
 def _interior_mask(bl):
     mk = np.zeros(bl.total, dtype=bool)
     for b in range(len(bl)):
-        n = bl.sz[b][0]
-        v = np.zeros((n, n, n), dtype=bool)
+        s = bl.sz[b]
+        v = np.zeros((s[2], s[1], s[0]), dtype=bool)
         v[3:-3, 3:-3, 3:-3] = True
-        mk[bl.offsets[b]:bl.offsets[b] + n ** 3] = v.reshape(-1)
+        mk[bl.offsets[b]:bl.offsets[b] + v.size] = v.reshape(-1)
     return mk
 
 
@@ -49,11 +49,16 @@ def test_ghost_fill_matches_oracle():
     np.testing.assert_array_equal(du.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("adv", ["upwind", "centred"])
-def test_rk4_step_matches_oracle(adv):
+@pytest.mark.parametrize("adv,n", [("upwind", 16), ("centred", 16), ("upwind", 17), ("upwind", 20)])
+def test_rk4_step_matches_oracle(adv, n):
+    """Two RK4 steps against the oracle stepper.  Interiors of 17 and 20
+    points are not multiples of the 8-point tile: the last tile along each
+    axis is shifted back and overlaps its neighbour, and the accumulator of
+    stages 1 and 2 (acc += dt/3 k) must still be updated once per point."""
     import pyoracle as po
     import sympygr_amd as sgr
-    torch, bl, u = _setup()
+    torch, bl, u = _setup(n)
+    print(f"rk4 step adv={adv} interior={n}", flush=True)
     plan = sgr.Plan(bl, sgr.params(adv=adv, ko_sigma=0.1))
     du = torch.from_numpy(u).cuda()
     sa, sb, acc = torch.zeros_like(du), torch.zeros_like(du), torch.zeros_like(du)
@@ -65,7 +70,7 @@ def test_rk4_step_matches_oracle(adv):
         torch.cuda.synchronize()
         ref = po.rk4_step(ref, bl, NB, DT, P, nthreads=4)
         e = _err(du.cpu().numpy(), ref, mk)
-        print(f"{adv} step {step} err {e:.2e}", flush=True)
+        print(f"{adv} n={n} step {step} err {e:.2e}", flush=True)
         assert e <= 1e-12
 
 
@@ -88,6 +93,55 @@ def test_rk4_graph_replays_match_eager():
         torch.cuda.synchronize()
         assert torch.equal(dg, de), f"replay {rep} differs"
     g.close()
+
+
+@pytest.mark.parametrize("n_int", [13, 25])
+def test_rk4_stages_on_a_shifted_tile_block(n_int):
+    """sgr_rk4_stage 0..3 one by one on a single block of 13 or 25 interior
+    points (Dendro's eleOrder*2^k+1 sizes, every axis with an overlapping last
+    tile), each stage against the tableau applied to the oracle RHS."""
+    import pyoracle as po
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    torch = __import__("torch")
+    n = n_int + 6
+    pmin, pmax = (-0.6, -0.5, -0.7), (-0.6 + 0.05 * (n - 1), -0.5 + 0.05 * (n - 1), -0.7 + 0.05 * (n - 1))
+    bl = data.BlockList()
+    bl.add((n, n, n), pmin, pmax, 0)
+    u0 = data.perturbed_flat_block(seed=9, n=n, amp=1e-2, pmin=pmin, pmax=pmax)
+    P = po.make_params(upwind=True, ko_sigma=0.1)
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", ko_sigma=0.1))
+    dt = 0.01
+    mk = _interior_mask(bl)
+    du0 = torch.from_numpy(u0).cuda()
+    acc = torch.full_like(du0, 3.0)
+    nxt = torch.zeros_like(du0)
+    stage_in = du0.clone()
+    ref_acc = np.full_like(u0, 3.0)
+    ref_stage = u0.copy()
+    for stage in range(4):
+        print(f"n_int={n_int} stage {stage}", flush=True)
+        k = po.rhs_block(ref_stage, (n, n, n), pmin, pmax, 0, P)
+        w = dt / (6.0 if stage in (0, 3) else 3.0)
+        cn = (dt / 2.0, dt / 2.0, dt)[stage] if stage < 3 else None
+        if stage < 3:
+            ref_acc[:, mk] = (u0[:, mk] if stage == 0 else ref_acc[:, mk]) + w * k[:, mk]
+            new_stage = u0.copy()
+            new_stage[:, mk] = u0[:, mk] + cn * k[:, mk]
+        else:
+            new_stage = ref_stage.copy()
+            new_stage[:, mk] = ref_acc[:, mk] + w * k[:, mk]
+        plan.rk4_stage(stage, dt, du0, stage_in, acc, nxt)
+        torch.cuda.synchronize()
+        got_next = nxt.cpu().numpy()
+        e = _err(got_next, new_stage, mk)
+        ea = _err(acc.cpu().numpy(), ref_acc, mk) if stage < 3 else 0.0
+        print(f"  next err {e:.2e} acc err {ea:.2e}", flush=True)
+        assert e <= 1e-12 and ea <= 1e-12
+        # next stage reads this stage's output (ghosts unchanged: one block, no refresh)
+        ref_stage = new_stage.copy()
+        ref_stage[:, ~mk] = u0[:, ~mk]
+        stage_in = torch.from_numpy(ref_stage).cuda()
 
 
 def test_rk4_stage_rejects_bad_arguments():
@@ -148,10 +202,12 @@ def test_chunked_workspace_matches_single_launch(monkeypatch):
     assert torch.equal(ca, cb)
 
 
-def test_octree_uneven_blocks_match_oracle():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_octree_uneven_blocks_match_oracle(precision):
     """Uneven octree leaves (interior 16/24/32/48, refined toward the
-    punctures, Brill-Lindquist data) in one plan; oracle on one block of each
-    size plus the boundary blocks' first."""
+    punctures, Brill-Lindquist data) in one plan; the FP64 oracle on one block
+    of each size plus the first boundary block of each size.  FP64 to 1e-12;
+    mixed precision (FP32 pointwise algebra) to 1e-6 normwise per field."""
     import pyoracle as po
     import sympygr_amd as sgr
     from sympygr_amd import data
@@ -159,7 +215,7 @@ def test_octree_uneven_blocks_match_oracle():
     bl = data.octree_blocks(lmax=4)
     u = data.fill_blocks(bl, backend="torch", device="cuda")
     out = torch.zeros_like(u)
-    plan = sgr.Plan(bl, sgr.params(adv="upwind"))
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=precision))
     plan.rhs(out, u)
     torch.cuda.synchronize()
     pick, seen = [], set()
@@ -177,8 +233,8 @@ def test_octree_uneven_blocks_match_oracle():
         g = got.reshape(24, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3].reshape(24, -1)
         r = ref.reshape(24, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3].reshape(24, -1)
         e = max(np.abs(g[f] - r[f]).max() / max(np.abs(r[f]).max(), 1e-300) for f in range(24))
-        print(f"octree block {b} size {s[0]} bflag {bl.bflag[b]} err {e:.2e}", flush=True)
-        assert e <= 1e-12
+        print(f"octree {precision} block {b} size {s[0]} bflag {bl.bflag[b]} err {e:.2e}", flush=True)
+        assert e <= (1e-12 if precision == "fp64" else 1e-6)
 
 
 def test_large_single_block_matches_sub_blocks():
@@ -215,3 +271,36 @@ def test_large_single_block_matches_sub_blocks():
         err = ((a - b).abs().amax(dim=(1, 2, 3)) / b.abs().amax(dim=(1, 2, 3)).clamp_min(1e-300)).max().item()
         print(f"sub-block at {(i0, j0, k0)} flags {flags} err {err:.2e}", flush=True)
         assert err <= 1e-12
+
+
+def test_check_finite_reports_an_injected_nan():
+    """A NaN in one ghost node of one block is reported as SGR_E_NONFINITE;
+    the clean input passes; an Inf is reported as well."""
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params())
+    du = torch.from_numpy(u).cuda()
+    assert plan.check_finite(du) == 0
+    ghost = int(np.flatnonzero(~_interior_mask(bl))[123])
+    du[7, ghost] = float("nan")
+    assert plan.check_finite(du) == -5
+    assert "non-finite" in sgr.last_error()
+    du[7, ghost] = u[7, ghost]
+    du[20, bl.offsets[-1] + 5] = float("inf")
+    assert plan.check_finite(du) == -5
+
+
+def test_short_or_foreign_tensors_are_rejected_on_the_host():
+    import sympygr_amd as sgr
+    torch, bl, u = _setup()
+    plan = sgr.Plan(bl, sgr.params())
+    du = torch.from_numpy(u).cuda()
+    with pytest.raises(ValueError, match="needs"):
+        plan.ghost_fill(du[:, :-1].contiguous())
+    with pytest.raises(ValueError, match="float64"):
+        plan.rhs(du.float(), du)
+    with pytest.raises(ValueError, match="cuda:0"):
+        plan.check_finite(torch.from_numpy(u))
+    c = torch.zeros((4, bl.total - 1), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="needs"):
+        plan.constraints(du, c)

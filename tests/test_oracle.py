@@ -231,3 +231,20 @@ def test_oracle_matches_compiled_reference_directly(bflag):
         ref = po.ref_rhs_block(var, u, sz, pmin, pmax, bflag, P)
         out = po.rhs_block(u, sz, pmin, pmax, bflag, P)
         assert normwise(interior(out, sz), interior(ref, sz)).max() <= 1e-13
+
+
+@pytest.mark.skipif(not po.ref_available("u"), reason="compiled reference not built")
+def test_reference_threaded_driver_equals_one_call_per_block():
+    """The compiled reference's OpenMP loop over blocks (as its example driver
+    runs it; the CPU baseline times it) gives exactly the per-block calls."""
+    from sympygr_amd import data
+    bl = data.uniform_tiling(nb=2, n=16, lo=-1.0, hi=1.0, stagger=0.5, faces=True)
+    u = data.fill_blocks(bl, kind="perturbed", seed=3, amp=1e-2, lo=-1.0, L=2.0)
+    P = po.make_params(upwind=True, ko_sigma=0.1)
+    threaded = po.ref_rhs_blocks("u", u, bl, P, nthreads=4)
+    for b in range(len(bl)):
+        o, s = bl.offsets[b], bl.sz[b]
+        n = s[0] * s[1] * s[2]
+        one = po.ref_rhs_block("u", u[:, o:o + n], s, bl.pmin[b], bl.pmax[b], bl.bflag[b], P)
+        np.testing.assert_array_equal(threaded[:, o:o + n].reshape(24, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3],
+                                      one.reshape(24, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3])
