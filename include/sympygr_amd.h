@@ -39,6 +39,12 @@ enum { SGR_ADV_CENTRED = 0, SGR_ADV_UPWIND = 1 };
 enum { SGR_PREC_FP64 = 0, SGR_PREC_MIXED = 1 };
 enum { SGR_GAUGE_STANDARD = 0, SGR_GAUGE_ROCHESTER = 1 };
 enum { SGR_ETA_CONST = 0, SGR_ETA_FUNC = 1 };
+enum { SGR_FORM_BSSN = 0, SGR_FORM_CCZ4 = 1 };
+
+/* Fields of a CCZ4 plan: the 24 BSSN fields (the Gt slots hold the evolved
+ * Gamma-hat^i) and Theta as field 24.  Every field-array argument of a CCZ4
+ * plan's entry points points to SGR_NUM_VARS_CCZ4 arrays. */
+#define SGR_NUM_VARS_CCZ4 25
 
 /* Physics and method parameters.  The reference reads these from mutable
  * bssn:: globals at call time (bssn/src/parameters.cpp:99-109,
@@ -65,10 +71,17 @@ typedef struct {
     uint32_t xi[3];           /* BSSN_XI (unsigned in the reference) */
     double eta_func_r0;       /* BSSN_ETA_R0 */
     double eta_power[2];      /* BSSN_ETA_POWER: p0, p1 */
+    /* The evolved system: SGR_FORM_BSSN (bssn/src/rhs.cpp) or SGR_FORM_CCZ4, the
+       conformal covariant Z4 system of the reference's ccz4_eqns.py:195-431
+       (its closures stated in sympygr_amd/codegen/ccz4_eqns.py; standard gauge
+       only). */
+    int32_t formulation;
+    double kappa[3];          /* CCZ4 damping kappa1, kappa2, kappa3 (real; the
+                                 reference declares them unsigned, ccz4_eqns.py:42-45) */
 } sgr_params;
 
 /* Defaults of bssn/src/parameters.cpp:50-51,99-109,123 with adv_mode/precision
- * given: standard gauge, constant eta. */
+ * given: standard gauge, constant eta, BSSN; kappa = (0.02, 0, 0.5). */
 void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision);
 
 /* One ghost-padded block: the per-block arguments of the reference

@@ -206,27 +206,30 @@ void kreiss_oliger(double *D, const double *u, double h, const Grid &g, unsigned
 // beta0..2, chi, gt0..5.
 const int D2_FIELDS[11] = {0, 6, 7, 8, 1, 12, 13, 14, 15, 16, 17};
 
-// Sommerfeld falloff and asymptotic value per field (bssn/src/rhs.cpp:261-320).
-const double BC_FALLOFF[24] = {1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2};
-const double BC_ASYM[24] = {1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0};
+// Sommerfeld falloff and asymptotic value per field (bssn/src/rhs.cpp:261-320);
+// the CCZ4 Theta (field 24) falls off like K: (1, 0).
+const double BC_FALLOFF[25] = {1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 1};
+const double BC_ASYM[25] = {1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0};
+
+constexpr int NFMAX = 25;  // BSSN: 24 fields; CCZ4: 25 (Theta last)
 
 struct Workspace {
     std::vector<double> buf;
-    double *grad[24][3];
-    double *agrad[24][3];
-    double *grad2[24][6];
-    void resize(std::size_t n, bool upwind)
+    double *grad[NFMAX][3];
+    double *agrad[NFMAX][3];
+    double *grad2[NFMAX][6];
+    void resize(std::size_t n, bool upwind, int nf = 24)
     {
-        const std::size_t narr = 72 + 66 + (upwind ? 72 : 0);
+        const std::size_t narr = 3 * nf + 66 + (upwind ? 3 * nf : 0);
         if (buf.size() < narr * n) buf.assign(narr * n, 0.0);
         double *p = buf.data();
-        for (int f = 0; f < 24; f++)
+        for (int f = 0; f < nf; f++)
             for (int d = 0; d < 3; d++) { grad[f][d] = p; p += n; }
-        for (int f = 0; f < 24; f++)
+        for (int f = 0; f < NFMAX; f++)
             for (int s = 0; s < 6; s++) grad2[f][s] = nullptr;
         for (int q = 0; q < 11; q++)
             for (int s = 0; s < 6; s++) { grad2[D2_FIELDS[q]][s] = p; p += n; }
-        for (int f = 0; f < 24; f++)
+        for (int f = 0; f < nf; f++)
             for (int d = 0; d < 3; d++) {
                 if (upwind) { agrad[f][d] = p; p += n; }
                 else agrad[f][d] = grad[f][d];
@@ -238,9 +241,10 @@ struct Workspace {
 // second derivatives and the mixed ones as compositions grad2_0_1 = D_y(grad_0),
 // grad2_0_2 = D_z(grad_0), grad2_1_2 = D_z(grad_1); advective derivatives along
 // d use beta^d (bssnrhs_derivs_adv.h).
-void sweep_all(Workspace &W, const double *const *in, const double *h, const Grid &g, unsigned bflag, bool upwind)
+void sweep_all(Workspace &W, const double *const *in, const double *h, const Grid &g, unsigned bflag, bool upwind,
+               int nf = 24)
 {
-    for (int f = 0; f < 24; f++)
+    for (int f = 0; f < nf; f++)
         for (int a = 0; a < 3; a++) first(W.grad[f][a], in[f], h[a], g, bflag, a);
     for (int q = 0; q < 11; q++) {
         const int f = D2_FIELDS[q];
@@ -252,7 +256,7 @@ void sweep_all(Workspace &W, const double *const *in, const double *h, const Gri
         first(W.grad2[f][4], W.grad[f][1], h[2], g, bflag, 2);
     }
     if (upwind)
-        for (int f = 0; f < 24; f++)
+        for (int f = 0; f < nf; f++)
             for (int a = 0; a < 3; a++) advective(W.agrad[f][a], in[f], h[a], g, in[6 + a], bflag, a);
 }
 
@@ -301,17 +305,19 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
     if (sz[0] < 13 || sz[1] < 13 || sz[2] < 13 || bflag > 63) return -2;
     const Grid g{(int)sz[0], (int)sz[1], (int)sz[2]};
     const std::size_t n = (std::size_t)g.nx * g.ny * g.nz;
-    const double *in[24];
-    double *out[24];
-    for (int f = 0; f < 24; f++) {
+    const int nf = P->ccz4 ? 25 : 24;
+    const double *in[NFMAX];
+    double *out[NFMAX];
+    for (int f = 0; f < nf; f++) {
         if (!inv[f] || !outv[f]) return -1;
         in[f] = inv[f] + offset;
         out[f] = outv[f] + offset;
     }
     const double h[3] = {(pmax[0] - pmin[0]) / (g.nx - 1), (pmax[1] - pmin[1]) / (g.ny - 1), (pmax[2] - pmin[2]) / (g.nz - 1)};
-    W.resize(n, P->upwind != 0);
+    W.resize(n, P->upwind != 0, nf);
 
-    sweep_all(W, in, h, g, bflag, P->upwind != 0);
+    sweep_all(W, in, h, g, bflag, P->upwind != 0, nf);
+    const double kappa1 = P->kappa[0], kappa2 = P->kappa[1], kappa3 = P->kappa[2];
 
     const double lambda0 = P->lambda[0], lambda1 = P->lambda[1], lambda2 = P->lambda[2], lambda3 = P->lambda[3];
     const double lambda_f0 = P->lambda_f[0], lambda_f1 = P->lambda_f[1];
@@ -348,7 +354,11 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 #define BSSN_ALT_ON rochester
 #define BSSN_OUT_ALT(f, v) out[f][pp] = (v)
 #define BSSN_BARRIER()
+                if (P->ccz4) {
+#include "gen/ccz4_algebra.inc"
+                } else {
 #include "gen/bssn_algebra.inc"
+                }
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
@@ -371,7 +381,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
             if (!(bflag & (1u << face))) continue;
             const int axis = face / 2;
             const int pos = (face & 1) ? g.n(axis) - 4 : 3;
-            for (int f = 0; f < 24; f++) {
+            for (int f = 0; f < nf; f++) {
                 for_box(g, b, axis, pos, [&](std::ptrdiff_t pp, int i, int j, int k) {
                     const double x = pmin[0] + i * h[0], y = pmin[1] + j * h[1], z = pmin[2] + k * h[2];
                     const double inv_r = 1.0 / std::sqrt(x * x + y * y + z * z);
@@ -385,7 +395,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
     // Kreiss-Oliger sweeps into the first-derivative arrays, then
     // rhs += sigma (KO_x + KO_y + KO_z) on the interior (rhs.cpp:324-369).
     const double sigma = P->ko_sigma;
-    for (int f = 0; f < 24; f++) {
+    for (int f = 0; f < nf; f++) {
         for (int a = 0; a < 3; a++) kreiss_oliger(W.grad[f][a], in[f], h[a], g, bflag, a);
         for_box(g, interior(g), 0, -1, [&](std::ptrdiff_t pp, int, int, int) {
             out[f][pp] += sigma * (W.grad[f][0][pp] + W.grad[f][1][pp] + W.grad[f][2][pp]);

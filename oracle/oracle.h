@@ -30,6 +30,9 @@ typedef struct {
     uint32_t xi[3];          /* BSSN_XI */
     double eta_func_r0;      /* BSSN_ETA_R0 */
     double eta_power[2];     /* BSSN_ETA_POWER */
+    int32_t ccz4;            /* 1: the CCZ4 system (sympygr_amd/codegen/ccz4_eqns.py):
+                                25 fields, Theta last; the arrays hold 25 pointers */
+    double kappa[3];         /* CCZ4 damping kappa1..3 */
 } oracle_params;
 
 /* One block, same argument meaning as the reference bssnrhs (rhs.h:33-36). */

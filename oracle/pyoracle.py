@@ -28,13 +28,18 @@ class OracleParams(ctypes.Structure):
         ("xi", ctypes.c_uint32 * 3),
         ("eta_func_r0", ctypes.c_double),
         ("eta_power", ctypes.c_double * 2),
+        ("ccz4", ctypes.c_int32),
+        ("kappa", ctypes.c_double * 3),
     ]
 
 
 def make_params(upwind=True, ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
                 lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
-                eta_func_r0=1.31, eta_power=(2.0, 2.0)):
+                eta_func_r0=1.31, eta_power=(2.0, 2.0), ccz4=False, kappa=(0.02, 0.0, 0.5)):
     p = OracleParams()
+    p.ccz4 = 1 if ccz4 else 0
+    for i in range(3):
+        p.kappa[i] = kappa[i]
     p.rochester = 1 if gauge == "rochester" else 0
     p.eta_func = 1 if eta == "func" else 0
     for i in range(3):

@@ -47,6 +47,8 @@ class SgrParams(ctypes.Structure):
         ("xi", ctypes.c_uint32 * 3),
         ("eta_func_r0", ctypes.c_double),
         ("eta_power", ctypes.c_double * 2),
+        ("formulation", ctypes.c_int32),
+        ("kappa", ctypes.c_double * 3),
     ]
 
 
@@ -144,13 +146,22 @@ def check(rc):
 
 GAUGE = {"standard": 0, "rochester": 1}
 ETA = {"const": 0, "func": 1}
+FORM = {"bssn": 0, "ccz4": 1}
+NUM_VARS_CCZ4 = 25
+VAR_NAMES_CCZ4 = VAR_NAMES + ["Theta"]
 
 
 def params(adv="upwind", precision="fp64", ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
            lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
-           eta_func_r0=1.31, eta_power=(2.0, 2.0)) -> SgrParams:
+           eta_func_r0=1.31, eta_power=(2.0, 2.0), formulation="bssn", kappa=(0.02, 0.0, 0.5)) -> SgrParams:
+    """sgr_params: the reference's parameter defaults with the given
+    choices.  formulation "ccz4" evolves the CCZ4 system (25 fields, Theta
+    last) with damping kappa = (kappa1, kappa2, kappa3)."""
     p = SgrParams()
     lib().sgr_params_default(ctypes.byref(p), ADV[adv], PREC[precision])
+    p.formulation = FORM[formulation]
+    for i in range(3):
+        p.kappa[i] = kappa[i]
     for i in range(4):
         p.lambda_[i] = lambda_[i]
     p.lambda_f[0], p.lambda_f[1] = lambda_f
@@ -235,6 +246,7 @@ class Plan:
         self._barr = block_array(blocks)
         check(lib().sgr_plan_create(ctypes.byref(self._h), self._barr, len(blocks), ctypes.byref(prm), device))
         self.device = device
+        self.nfields = NUM_VARS_CCZ4 if prm.formulation == FORM["ccz4"] else NUM_VARS
 
     def info(self) -> SgrPlanInfo:
         i = SgrPlanInfo()
@@ -242,8 +254,9 @@ class Plan:
         return i
 
     def _f(self, t):
-        """Checked field pointers of a (24, total) tensor for this plan."""
-        return _field_ptrs(t, self.blocks.total, self.device)
+        """Checked field pointers of a (nfields, total) tensor for this plan
+        (24 fields, or 25 for CCZ4)."""
+        return _rows(t, self.nfields, self.blocks.total, self.device)
 
     def rhs(self, out, u, stream=None):
         """Asynchronous RHS on `stream` (default: torch's current stream)."""

@@ -30,7 +30,11 @@ import re
 import sympy as sp
 from sympy.printing.c import C99CodePrinter
 
-from .bssn_eqs import FIELD_INDEX, SYM
+from .bssn_eqs import FIELD_INDEX as _BSSN_FIELD_INDEX, SYM
+
+# Field indices of the unzipped arrays: bssn::VAR order, and Theta after
+# them in the CCZ4 system (field 24).
+FIELD_INDEX = dict(_BSSN_FIELD_INDEX, Theta=24)
 
 _D1 = re.compile(r"^grad_(\d)_(\w+)$")
 _D2 = re.compile(r"^grad2_(\d)_(\d)_(\w+)$")

@@ -62,7 +62,10 @@ AxisConst axis_const(double pmin, double pmax, uint32_t n)
 
 // Workspace bytes per 8^3 tile: RHS pass (staged derivatives in W plus the
 // double tails) and constraint pass (its derivative slots plus KO sums).
-uint64_t rhs_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (RS_NSLOT * wsz + RS_NTAIL * sizeof(double)); }
+uint64_t rhs_tile_bytes(size_t wsz, int nf)
+{
+    return (uint64_t)TILE_PTS * (rs_nslot(nf) * wsz + nf * sizeof(double));
+}
 uint64_t con_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (NSLOT * wsz + NF * sizeof(double)); }
 
 int validate_params(const sgr_params *p)
@@ -76,6 +79,12 @@ int validate_params(const sgr_params *p)
         return fail(SGR_E_INVALID, "gauge must be SGR_GAUGE_STANDARD or SGR_GAUGE_ROCHESTER");
     if (p->eta_mode != SGR_ETA_CONST && p->eta_mode != SGR_ETA_FUNC)
         return fail(SGR_E_INVALID, "eta_mode must be SGR_ETA_CONST or SGR_ETA_FUNC");
+    if (p->formulation != SGR_FORM_BSSN && p->formulation != SGR_FORM_CCZ4)
+        return fail(SGR_E_INVALID, "formulation must be SGR_FORM_BSSN or SGR_FORM_CCZ4");
+    if (p->formulation == SGR_FORM_CCZ4 && p->gauge != SGR_GAUGE_STANDARD)
+        return fail(SGR_E_UNSUPPORTED, "CCZ4 is evolved with the standard gauge only");
+    for (int i = 0; i < 3; i++)
+        if (!std::isfinite(p->kappa[i])) return fail(SGR_E_INVALID, "kappa must be finite");
     return SGR_OK;
 }
 
@@ -115,17 +124,18 @@ Params device_params(const sgr_params &p)
     q.eta_func_r0 = p.eta_func_r0;
     q.eta_power[0] = p.eta_power[0];
     q.eta_power[1] = p.eta_power[1];
+    for (int i = 0; i < 3; i++) q.kappa[i] = p.kappa[i];
     return q;
 }
 
-__global__ void k_check_finite(FieldPtrs in, const BlockDesc *blocks, uint32_t nblocks, int *flag)
+__global__ void k_check_finite(FieldPtrs in, int nf, const BlockDesc *blocks, uint32_t nblocks, int *flag)
 {
     for (uint32_t b = blockIdx.y; b < nblocks; b += gridDim.y) {
         const BlockDesc &B = blocks[b];
         const uint64_t n = (uint64_t)B.n[0] * B.n[1] * B.n[2];
         for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
             bool bad = false;
-            for (int f = 0; f < NF; f++) bad |= !isfinite(in.p[f][B.offset + e]);
+            for (int f = 0; f < nf; f++) bad |= !isfinite(in.p[f][B.offset + e]);
             if (bad) atomicOr(flag, 1);
         }
     }
@@ -149,8 +159,8 @@ struct sgr_plan {
     uint32_t *d_tile_block = nullptr;
     uint32_t ntiles = 0;
     // One device workspace, tile-major, shared by the two passes that use it:
-    //   RHS:         chunk_tiles tiles of RS_NSLOT W values per point, then
-    //                their RS_NTAIL double tails per point
+    //   RHS:         chunk_tiles tiles of rs_nslot(nf) W values per point,
+    //                then their nf double tails per point
     //   constraints: cchunk_tiles tiles of NSLOT W values per point, then
     //                their NF double KO sums per point
     void *d_ws = nullptr;
@@ -160,6 +170,7 @@ struct sgr_plan {
     uint64_t ws_bytes = 0;
     uint64_t interior = 0;
     size_t wsz = sizeof(double); // bytes of one staged derivative (W)
+    int nf = NF;                 // fields: NF (BSSN) or NF_CCZ4
     // ghost refresh table
     GhostTable ghost;
     GhostPatch *d_ghost = nullptr;
@@ -202,6 +213,10 @@ void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision)
     p->eta_func_r0 = 1.31;
     p->eta_power[0] = 2.0;
     p->eta_power[1] = 2.0;
+    p->formulation = SGR_FORM_BSSN;
+    p->kappa[0] = 0.02;
+    p->kappa[1] = 0.0;
+    p->kappa[2] = 0.5;
 }
 
 int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, const sgr_params *params, int device)
@@ -215,6 +230,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     if (!p) return fail(SGR_E_NOMEM, "host allocation failed");
     p->device = device;
     p->params = *params;
+    p->nf = params->formulation == SGR_FORM_CCZ4 ? NF_CCZ4 : NF;
     // SGR_DERIVS_THREADS=256|512 sets the derivative kernel's workgroup size.
     {
         const char *tv = getenv("SGR_DERIVS_THREADS");
@@ -259,7 +275,7 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     // in as many launches as that same allocation needs (at least one tile).
     const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
     p->wsz = wsz;
-    const uint64_t rhs_tile = rhs_tile_bytes(wsz), con_tile = con_tile_bytes(wsz);
+    const uint64_t rhs_tile = rhs_tile_bytes(wsz, p->nf), con_tile = con_tile_bytes(wsz);
     {
         double gib = 40.0;
         const char *gv = getenv("SGR_WS_GIB");
@@ -355,7 +371,7 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
     info->tiles = plan->ntiles;
     info->interior_points = plan->interior;
     info->workspace_bytes = plan->ws_bytes;
-    info->derivative_slots = RS_NSLOT + RS_NTAIL;
+    info->derivative_slots = rs_nslot(plan->nf) + plan->nf;
     info->chunks = plan->chunk_tiles ? (plan->ntiles + plan->chunk_tiles - 1) / plan->chunk_tiles : 0;
     return SGR_OK;
 }
@@ -365,7 +381,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
 {
     if (!plan || !out || !in) return fail(SGR_E_INVALID, "null argument");
     LaunchArgs a{};
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < plan->nf; f++) {
         if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
         a.in.p[f] = in[f];
         a.out.p[f] = out[f];
@@ -378,6 +394,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.params = device_params(plan->params);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
+    a.ccz4 = plan->nf == NF_CCZ4;
     a.derivs_threads = plan->derivs_threads;
     for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
     a.rk.stage = -1;
@@ -459,14 +476,14 @@ int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t st
     if (!plan || !u) return fail(SGR_E_INVALID, "null argument");
     if (!plan->ghost.complete) return fail(SGR_E_UNSUPPORTED, plan->ghost.msg);
     OutPtrs o;
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < plan->nf; f++) {
         if (!u[f]) return fail(SGR_E_INVALID, "null field pointer");
         o.p[f] = u[f];
     }
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = launch_ghost_fill(o, plan->d_ghost, plan->d_gbox, plan->ghost.bstart, stream);
+    hipError_t e = launch_ghost_fill(o, plan->nf, plan->d_ghost, plan->d_gbox, plan->ghost.bstart, stream);
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "ghost fill launch");
     return SGR_OK;
@@ -514,7 +531,7 @@ int sgr_rk4_stage(sgr_plan *plan, int stage, double dt, const double *const u_n[
     const double w[4] = {dt / 6, dt / 3, dt / 3, dt / 6};
     rk.cn = stage < 3 ? cn[stage] : 0.0;
     rk.w = w[stage];
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < plan->nf; f++) {
         if (!u_n[f] || !acc[f] || !u_next[f]) return fail(SGR_E_INVALID, "null field pointer");
         rk.u0.p[f] = u_n[f];
         rk.acc.p[f] = acc[f];
@@ -604,7 +621,7 @@ int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipSt
 {
     if (!plan || !in) return fail(SGR_E_INVALID, "null argument");
     FieldPtrs fp;
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < plan->nf; f++) {
         if (!in[f]) return fail(SGR_E_INVALID, "null field pointer");
         fp.p[f] = in[f];
     }
@@ -616,8 +633,8 @@ int sgr_check_finite(sgr_plan *plan, const double *const in[SGR_NUM_VARS], hipSt
     hipError_t e = hipMemsetAsync(plan->d_flag, 0, sizeof(int), stream);
     if (e == hipSuccess) {
         const uint32_t nb = (uint32_t)plan->hblocks.size();
-        hipLaunchKernelGGL(k_check_finite, dim3(64, nb < 65535 ? nb : 65535), dim3(256), 0, stream, fp, plan->d_blocks,
-                           nb, plan->d_flag);
+        hipLaunchKernelGGL(k_check_finite, dim3(64, nb < 65535 ? nb : 65535), dim3(256), 0, stream, fp, plan->nf,
+                           plan->d_blocks, nb, plan->d_flag);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&h, plan->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream);
@@ -688,7 +705,8 @@ int stage_plan(const sgr_block &b, const sgr_params *params, int device)
         if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(block)");
     }
     const bool same = S.plan && S.sz[0] == b.sz[0] && S.sz[1] == b.sz[1] && S.sz[2] == b.sz[2] &&
-                      S.adv == params->adv_mode && S.prec == params->precision;
+                      S.adv == params->adv_mode && S.prec == params->precision &&
+                      S.plan->params.formulation == params->formulation;
     if (!same) {
         if (S.plan) {
             hipStreamSynchronize(S.stream);
@@ -789,6 +807,8 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
     sgr_block b;
     int rc = block_checks(pmin, pmax, sz, bflag, params, 0, b);  // staged at offset 0
     if (rc) return rc;
+    if (params->formulation != SGR_FORM_BSSN)
+        return fail(SGR_E_UNSUPPORTED, "the host-array entry (bssnrhs) evaluates the BSSN system only");
     const uint64_t n = (uint64_t)sz[0] * sz[1] * sz[2];
     int cur = 0;
     hipGetDevice(&cur);

@@ -6,7 +6,9 @@
 
 namespace sgr {
 
-constexpr int NF = 24;     // evolved fields, order of bssn/include/grDef.h:31
+constexpr int NF = 24;     // evolved BSSN fields, order of bssn/include/grDef.h:31
+constexpr int NF_CCZ4 = 25;  // CCZ4: the BSSN fields (Gt holding Gamma-hat) and Theta last
+constexpr int NFMAX = NF_CCZ4;
 constexpr int GW = 3;      // ghost width (bssn/include/block.h:21)
 constexpr int T = 8;       // tile edge (interior points)
 constexpr int TH = T + 2 * GW;  // tile edge with halo
@@ -70,10 +72,11 @@ __host__ __device__ constexpr int sym_slot(int j, int k)
 // Rochester gauge, xi1 for the shift, 1 otherwise) and for B^i
 // (1 - lambda3) beta.ad Gt^i + lambda2 beta.ad B^i + sigma KO B^i
 // (sigma KO B^i alone in the Rochester gauge, where B is not evolved).
+// CCZ4 adds the first derivatives of Theta (slots 69..71) and its tail.
 constexpr int RS_NSLOT = 69;
-constexpr int RS_NTAIL = NF;
-__host__ __device__ constexpr bool rs_has_d1(int f) { return f <= 8 || (f >= 12 && f <= 17); }
-__host__ __device__ constexpr int rs_d1(int f, int d) { return 3 * (f <= 8 ? f : f - 3) + d; }
+__host__ __device__ constexpr int rs_nslot(int nfld) { return nfld == NF_CCZ4 ? RS_NSLOT + 3 : RS_NSLOT; }
+__host__ __device__ constexpr bool rs_has_d1(int f) { return f <= 8 || (f >= 12 && f <= 17) || f == 24; }
+__host__ __device__ constexpr int rs_d1(int f, int d) { return f == 24 ? RS_NSLOT + d : 3 * (f <= 8 ? f : f - 3) + d; }
 __host__ __device__ constexpr int rs_d2(int f, int s) { return (f == 1 ? 51 : 45) + s; }
 __host__ __device__ constexpr int rs_lap(int f) { return f <= 8 ? 57 + (f - 6) : 60 + (f - 12); }
 __host__ __device__ constexpr int rs_gdb(int j) { return 66 + j; }
@@ -155,10 +158,11 @@ __host__ __device__ inline double metric_contract(const double ig[6], const doub
            ig[5] * h[5];
 }
 
-// Sommerfeld falloff / asymptotic value per field (bssn/src/rhs.cpp:261-320).
+// Sommerfeld falloff / asymptotic value per field (bssn/src/rhs.cpp:261-320);
+// the CCZ4 Theta (24) falls off like K.
 __host__ __device__ constexpr double bc_falloff(int f)
 {
-    return (f >= 3 && f <= 5) || f >= 18 ? 2.0 : 1.0;
+    return (f >= 3 && f <= 5) || (f >= 18 && f <= 23) ? 2.0 : 1.0;
 }
 __host__ __device__ constexpr double bc_asym(int f)
 {
@@ -201,6 +205,7 @@ struct Params {
     double xi[3];        // BSSN_XI
     double eta_func_r0;  // BSSN_ETA_R0
     double eta_power[2]; // BSSN_ETA_POWER
+    double kappa[3];     // CCZ4 damping kappa1..3
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -216,10 +221,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n)
 }
 
 struct FieldPtrs {
-    const double *p[NF];
+    const double *p[NFMAX];
 };
 struct OutPtrs {
-    double *p[NF];
+    double *p[NFMAX];
 };
 
 }  // namespace sgr

@@ -32,6 +32,7 @@ struct LaunchArgs {
     Params params;
     bool upwind;
     bool mixed;
+    bool ccz4;                   // the CCZ4 system (25 fields) instead of BSSN
     hipEvent_t ev[4];            // optional: start/stop of k_stencil, start/stop of k_pointwise
     RkArgs rk;                   // rk.stage < 0: plain RHS
     int derivs_threads;          // constraint pass: k_derivs workgroup size, 256 or 512
@@ -41,7 +42,7 @@ struct LaunchArgs {
 hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s);
 
 // boxes sorted by class: class c occupies [bstart[c], bstart[c+1])
-hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes,
+hipError_t launch_ghost_fill(OutPtrs u, int nf, const GhostPatch *patches, const GhostBox *boxes,
                              const uint32_t bstart[GHOST_CLASSES + 1], hipStream_t s);
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);    // derivative workspace + pointwise

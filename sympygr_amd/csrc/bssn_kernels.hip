@@ -444,7 +444,8 @@ __device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, i
 // output's tail (its advection and Kreiss-Oliger terms, in double); on a
 // flagged-face point the tail is the whole output, the Sommerfeld value
 // (bssn/src/rhs.cpp:833-977) plus sigma KO.
-template <typename W, bool UPWIND>
+// NFLD: 24 (BSSN) or 25 (CCZ4: Theta last, with its first derivatives staged).
+template <typename W, bool UPWIND, int NFLD>
 __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                      double *__restrict__ wsT, Params P, uint32_t tile0)
@@ -496,8 +497,8 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
         for (int q = 0; q < 6; q++) g[q] = valid ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
         inverse_metric(g, igp);
     }
-    W *__restrict__ wsb = ws + ws_tile_base(ct, RS_NSLOT);
-    double *__restrict__ tb = wsT + ws_tile_base(ct, RS_NTAIL);
+    W *__restrict__ wsb = ws + ws_tile_base(ct, rs_nslot(NFLD));
+    double *__restrict__ tb = wsT + ws_tile_base(ct, NFLD);
     constexpr uint64_t SS = WS_SS;
 
     // halo element -> offset in the block (or -1 outside the padded block)
@@ -518,11 +519,11 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
     __syncthreads();
 
 #pragma unroll 1
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < NFLD; f++) {
         // fetch the next field's halo tile into registers; it lands in LDS
         // after this field's stencils, so the HBM latency overlaps them
         double pf[NPF];
-        if (f + 1 < NF) {
+        if (f + 1 < NFLD) {
             const double *__restrict__ sn = in.p[f + 1] + B.offset;
 #pragma unroll
             for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
@@ -671,7 +672,7 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
             }
         }
         __syncthreads();  // every read of U (and DX, DY) for field f is done
-        if (f + 1 < NF) {
+        if (f + 1 < NFLD) {
 #pragma unroll
             for (int q = 0; q < NPF; q++)
                 if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = pf[q];
@@ -680,6 +681,7 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
     }
 }
 
+#define SGR_PW_NFLD NF
 #define SGR_PW_NAME k_pointwise_g1
 #define SGR_PW_ALG "gen/bssn_algebra_g1.inc"
 #include "pointwise_body.h"
@@ -690,19 +692,44 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
 #include "pointwise_body.h"
 #undef SGR_PW_NAME
 #undef SGR_PW_ALG
+#undef SGR_PW_NFLD
+// the CCZ4 system (gen/ccz4_algebra_g*.inc, sympygr_amd/codegen/ccz4_eqns.py)
+#define SGR_PW_NFLD NF_CCZ4
+#define SGR_PW_NAME k_ccz4_pointwise_g1
+#define SGR_PW_ALG "gen/ccz4_algebra_g1.inc"
+#include "pointwise_body.h"
+#undef SGR_PW_NAME
+#undef SGR_PW_ALG
+#define SGR_PW_NAME k_ccz4_pointwise_g2
+#define SGR_PW_ALG "gen/ccz4_algebra_g2.inc"
+#include "pointwise_body.h"
+#undef SGR_PW_NAME
+#undef SGR_PW_ALG
+#undef SGR_PW_NFLD
 
 // ------------------------------------------------------------------ launch
 template <typename R, typename W, bool UPWIND, bool RK>
 static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
 {
     W *ws = (W *)a.ws;
+    const dim3 g(2 * a.ntiles), b(256);
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
-    hipExtLaunchKernelGGL((k_stencil<W, UPWIND>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
+    if (a.ccz4) {
+        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NF_CCZ4>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0,
+                              a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipExtLaunchKernelGGL((k_ccz4_pointwise_g1<R, W, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
+                              a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
+        hipExtLaunchKernelGGL((k_ccz4_pointwise_g2<R, W, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out,
+                              a.blocks, a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
+        return hipGetLastError();
+    }
+    hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NF>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
                           a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 g(2 * a.ntiles), b(256);
     hipExtLaunchKernelGGL((k_pointwise_g1<R, W, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
     hipExtLaunchKernelGGL((k_pointwise_g2<R, W, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
@@ -853,7 +880,7 @@ struct Idx3 {
 // NX*NY*NZ to NX*Ry*Rz/(ey*ez).  Sources are interior nodes and targets
 // ghost nodes, so boxes never read what another box writes.
 template <int NX, int NY, int NZ>
-__global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, const GhostPatch *__restrict__ patches,
+__global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, int nf, const GhostPatch *__restrict__ patches,
                                                    const GhostBox *__restrict__ boxes)
 {
     SGR_EXACT
@@ -904,7 +931,7 @@ __global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, const GhostPatch *
     double *T1 = T, *T2 = T + n1;
     const uint64_t sy = g.sn0, sz = (uint64_t)g.sn0 * g.sn1;
 #pragma unroll 1
-    for (int f = 0; f < NF; f++) {
+    for (int f = 0; f < nf; f++) {
         const double *__restrict__ v = u.p[f] + g.src_off;
         {
             Idx3 it(t, 256, ex, Ry);
@@ -946,7 +973,7 @@ __global__ __launch_bounds__(256) void k_ghost_sep(OutPtrs u, const GhostPatch *
 
 // Boxes whose nodes coincide with source nodes on every axis: copies, written
 // as the per-node form evaluates them (0.0 + 1.0*u, which maps -0.0 to +0.0).
-__global__ __launch_bounds__(256) void k_ghost_copy(OutPtrs u, const GhostPatch *__restrict__ patches,
+__global__ __launch_bounds__(256) void k_ghost_copy(OutPtrs u, int nf, const GhostPatch *__restrict__ patches,
                                                     const GhostBox *__restrict__ boxes)
 {
     SGR_EXACT
@@ -964,34 +991,34 @@ __global__ __launch_bounds__(256) void k_ghost_copy(OutPtrs u, const GhostPatch 
         const uint64_t src = g.src_off + ax + sy * ay + sz * az;
         const uint64_t dst = g.dst_off + x + dy * y + dz * z;
 #pragma unroll 4
-        for (int f = 0; f < NF; f++) u.p[f][dst] = 0.0 + 1.0 * u.p[f][src];
+        for (int f = 0; f < nf; f++) u.p[f][dst] = 0.0 + 1.0 * u.p[f][src];
     }
 }
 
 template <int CLS>
-static void launch_ghost_class(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes, const uint32_t *bstart,
-                               hipStream_t s)
+static void launch_ghost_class(OutPtrs u, int nf, const GhostPatch *patches, const GhostBox *boxes,
+                               const uint32_t *bstart, hipStream_t s)
 {
     const uint32_t n = bstart[CLS + 1] - bstart[CLS];
     if (n == 0) return;
     constexpr int NX = (CLS & 1) ? 1 : 6, NY = (CLS & 2) ? 1 : 6, NZ = (CLS & 4) ? 1 : 6;
     if constexpr (CLS == GHOST_CLASSES - 1)
-        hipLaunchKernelGGL(k_ghost_copy, dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
+        hipLaunchKernelGGL(k_ghost_copy, dim3(n), dim3(256), 0, s, u, nf, patches, boxes + bstart[CLS]);
     else
-        hipLaunchKernelGGL((k_ghost_sep<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, patches, boxes + bstart[CLS]);
+        hipLaunchKernelGGL((k_ghost_sep<NX, NY, NZ>), dim3(n), dim3(256), 0, s, u, nf, patches, boxes + bstart[CLS]);
 }
 
-hipError_t launch_ghost_fill(OutPtrs u, const GhostPatch *patches, const GhostBox *boxes,
+hipError_t launch_ghost_fill(OutPtrs u, int nf, const GhostPatch *patches, const GhostBox *boxes,
                              const uint32_t bstart[GHOST_CLASSES + 1], hipStream_t s)
 {
-    launch_ghost_class<0>(u, patches, boxes, bstart, s);
-    launch_ghost_class<1>(u, patches, boxes, bstart, s);
-    launch_ghost_class<2>(u, patches, boxes, bstart, s);
-    launch_ghost_class<3>(u, patches, boxes, bstart, s);
-    launch_ghost_class<4>(u, patches, boxes, bstart, s);
-    launch_ghost_class<5>(u, patches, boxes, bstart, s);
-    launch_ghost_class<6>(u, patches, boxes, bstart, s);
-    launch_ghost_class<7>(u, patches, boxes, bstart, s);
+    launch_ghost_class<0>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<1>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<2>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<3>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<4>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<5>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<6>(u, nf, patches, boxes, bstart, s);
+    launch_ghost_class<7>(u, nf, patches, boxes, bstart, s);
     return hipGetLastError();
 }
 

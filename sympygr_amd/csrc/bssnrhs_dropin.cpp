@@ -67,6 +67,8 @@ sgr_params current_params()
     p.eta_func_r0 = bssn::BSSN_ETA_R0;
     p.eta_power[0] = bssn::BSSN_ETA_POWER[0];
     p.eta_power[1] = bssn::BSSN_ETA_POWER[1];
+    p.formulation = SGR_FORM_BSSN;  // the reference's bssnrhs evolves BSSN
+    p.kappa[0] = p.kappa[1] = p.kappa[2] = 0.0;
     return p;
 }
 

@@ -28,8 +28,8 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
                              ((bf & 4u) && j == GW) || ((bf & 8u) && j == n1 - GW - 1) ||
                              ((bf & 16u) && k == GW) || ((bf & 32u) && k == n2 - GW - 1));
     // this tile's workspace (layout: ws_tile_base, slot stride WS_SS)
-    const W *__restrict__ wsb = ws + ws_tile_base(wg >> 1, RS_NSLOT) + l;
-    const double *__restrict__ tb = wsT + ws_tile_base(wg >> 1, RS_NTAIL) + l;
+    const W *__restrict__ wsb = ws + ws_tile_base(wg >> 1, rs_nslot(SGR_PW_NFLD)) + l;
+    const double *__restrict__ tb = wsT + ws_tile_base(wg >> 1, SGR_PW_NFLD) + l;
     constexpr uint64_t SS = WS_SS;
     const uint64_t pp = B.offset + i + (uint64_t)n0 * (j + (uint64_t)n1 * k);
 
@@ -49,7 +49,9 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
     const R lambda_f0 = (R)P.lambda_f[0], lambda_f1 = (R)P.lambda_f[1];
     const R eta = (R)eta_d;
     const R xi1 = (R)P.xi[1], xi2 = (R)P.xi[2];
+    const R kappa1 = (R)P.kappa[0], kappa2 = (R)P.kappa[1], kappa3 = (R)P.kappa[2];
     (void)lambda0, (void)lambda1, (void)lambda2, (void)lambda3, (void)lambda_f0, (void)lambda_f1, (void)xi1, (void)xi2;
+    (void)kappa1, (void)kappa2, (void)kappa3;
     const bool rochester = P.rochester;
 
 #define BSSN_U(F_) ((R)in.p[F_][pp])

@@ -16,6 +16,10 @@ Workloads
 * ``uniform_tiling`` -- 8x8x8 blocks of 32^3 interior points tiling
   [-8,8]^3 (dx = 0.0625), bflag on the outer faces.
 * ``octree_blocks`` -- an uneven block set refined toward the punctures.
+* ``ccz4_fill`` (``fill_blocks(kind="ccz4")``) -- the CCZ4 test data: the
+  perturbed flat background (amplitude 1e-2), Theta perturbed the same way
+  (field 24), and a Gaussian lapse pulse alpha += A exp(-r^2/w^2), A = 0.1,
+  w = 1.
 
 Random numbers come from SplitMix64 keyed by (seed, field, mode), so any
 implementation of the recipe reproduces the inputs bit for bit.
@@ -63,7 +67,7 @@ def fourier_modes(seed: int, f: int, nmodes: int = 8):
 class _Np:
     """numpy array-module shim"""
     def __init__(self):
-        self.sin, self.sqrt, self.maximum = np.sin, np.sqrt, np.maximum
+        self.sin, self.sqrt, self.maximum, self.exp = np.sin, np.sqrt, np.maximum, np.exp
 
     @staticmethod
     def arange(n, dtype=None, like=None):
@@ -87,7 +91,7 @@ class _Torch:
         import torch
         self.t = torch
         self.device = device
-        self.sin, self.sqrt = torch.sin, torch.sqrt
+        self.sin, self.sqrt, self.exp = torch.sin, torch.sqrt, torch.exp
 
     def maximum(self, a, b):
         if not hasattr(b, "shape"):
@@ -328,24 +332,41 @@ def perturbed_flat_fill(xp, x, y, z, out, seed=1, amp=1e-3, lo=-1.0, L=2.0):
         out[AT[s]] = out[AT[s]] - g[s] * tr / 3.0
 
 
+def ccz4_fill(xp, x, y, z, out, seed=1, amp=1e-2, lo=-1.0, L=2.0, pulse=0.1, width=1.0):
+    """CCZ4 data (25 fields, Theta last): perturbed_flat_fill for the 24 BSSN
+    fields (the Gt slots hold Gamma-hat), Theta = amp * 8 seeded Fourier
+    modes (key field 24), and the lapse pulse alpha += pulse exp(-r^2/width^2)."""
+    perturbed_flat_fill(xp, x, y, z, out, seed=seed, amp=amp, lo=lo, L=L)
+    acc = 0.0 * x
+    for (kx, ky, kz, ph) in fourier_modes(seed, 24):
+        acc = acc + xp.sin(2.0 * math.pi * (kx * (x - lo) + ky * (y - lo) + kz * (z - lo)) / L + ph)
+    out[24] = amp * acc
+    out[ALPHA] = out[ALPHA] + pulse * xp.exp(-(x * x + y * y + z * z) / (width * width))
+
+
 def fill_blocks(bl: BlockList, backend="numpy", device="cpu", kind="bl", **kw):
     """Allocate a (24, total) float64 array and fill every padded block with
     the analytic data (ghosts included): kind "bl" (Brill-Lindquist) or
-    "perturbed" (perturbed_flat_fill, keywords seed, amp, lo, L)."""
+    "perturbed" (perturbed_flat_fill, keywords seed, amp, lo, L); or a
+    (25, total) array for kind "ccz4" (ccz4_fill, same keywords plus pulse,
+    width)."""
     xp = _xp(backend, device)
-    u = xp.zeros((NFIELDS, bl.total))
+    nfields = 25 if kind == "ccz4" else NFIELDS
+    u = xp.zeros((nfields, bl.total))
     for b in range(len(bl)):
         sz = bl.sz[b]
         x, y, z = block_coords(xp, sz, bl.pmin[b], bl.pmax[b])
         n = sz[0] * sz[1] * sz[2]
-        vals = [None] * NFIELDS
+        vals = [None] * nfields
         if kind == "bl":
             brill_lindquist_fill(xp, x, y, z, vals)
         elif kind == "perturbed":
             perturbed_flat_fill(xp, x, y, z, vals, **kw)
+        elif kind == "ccz4":
+            ccz4_fill(xp, x, y, z, vals, **kw)
         else:
             raise ValueError(kind)
         o = bl.offsets[b]
-        for f in range(NFIELDS):
+        for f in range(nfields):
             u[f, o:o + n] = vals[f].reshape(-1)
     return u

