@@ -347,6 +347,58 @@ def octree_section(args, hev):
     return res, drift
 
 
+def ccz4_section(hev, steps=10):
+    """CCZ4 (25 fields, Theta last; sympygr_amd/codegen/ccz4_eqns.py) on one
+    block of 192^3 interior points (198^3 padded): the perturbed flat
+    background at amplitude 1e-2 plus a Gaussian lapse pulse (A = 0.1,
+    width 1), kappa1 = 0.02, kappa2 = 0, kappa3 = 0.5; RHS in FP64 and mixed
+    with the mixed error, and the constraint kernel.  Algorithmic bytes and
+    flops are the BSSN figures times 25/24."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    nL = 198
+    bl = data.BlockList()
+    bl.add((nL, nL, nL), (-4.0,) * 3, (4.0,) * 3, 63)
+    u = data.fill_blocks(bl, backend="torch", device="cuda", kind="ccz4", seed=7, amp=1e-2, lo=-4.0, L=8.0)
+    pts = bl.interior_points()
+    res = {"workload": "one block of 192^3 interior (198^3 padded), CCZ4, perturbed flat data (1e-2) plus a lapse "
+                       "pulse 0.1 exp(-r^2), all faces flagged, upwind, KO sigma 0.1, kappa = (0.02, 0, 0.5)",
+           "interior_points": pts}
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1, formulation="ccz4",
+                                       kappa=(0.02, 0.0, 0.5)))
+        outs[prec] = torch.zeros_like(u)
+        w, kd, kp = timed_steps(plan, outs[prec], u, steps, 2, hev)
+        t = (kd + kp) * 1e-3
+        rf = mix_roofline({nL - 6: pts}, t, prec)
+        rf = {k: (v * 25 / 24 if k in ("frac", "hbm_frac", "valu_frac") else v) for k, v in rf.items()}
+        rf["bound_mpts"] = rf["bound_mpts"] * 24 / 25
+        res[prec] = {"value": pts * steps / w / 1e6, "ms_per_step": 1e3 * w / steps,
+                     "kernel_ms": {"k_stencil": kd, "k_pointwise": kp}, "roofline": rf}
+        log(f"ccz4 {prec}: {res[prec]['value']:.1f} Mpts/s (k_stencil {kd:.2f} ms, pointwise {kp:.2f} ms)")
+        if prec == "fp64":
+            c = torch.zeros((4, bl.total), dtype=torch.float64, device="cuda")
+            plan.constraints(u, c)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                plan.constraints(u, c)
+            torch.cuda.synchronize()
+            tc = (time.perf_counter() - t0) / 3
+            res["constraints"] = {"ms": 1e3 * tc, "mpts": pts / tc / 1e6}
+            del c
+        del plan
+    err = normwise_err(outs["mixed"], outs["fp64"])
+    res["mixed"]["max_normwise_rel_err_vs_fp64"] = err.max().item()
+    res["mixed"]["per_field_normwise_rel_err"] = [float(f"{x:.3e}") for x in err.tolist()]
+    res["mixed_speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    del outs, u
+    torch.cuda.empty_cache()
+    return res
+
+
 def host_dropin_rate(bl, u, nsample=16, reps=3):
     """The C++ drop-in path on HOST arrays (sgr_bssnrhs_host, as bssnrhs(...)
     is called by a Dendro driver): one call per block over the first
@@ -486,6 +538,7 @@ def main():
                     help="leaf interior sizes of the octree section, drawn per leaf")
     ap.add_argument("--no-octree", action="store_true",
                     help="skip the batched uneven-octree measurements (RHS, RK4 step, drift) and the large block")
+    ap.add_argument("--no-ccz4", action="store_true", help="skip the CCZ4 192^3-block leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (k_hbm_copy, reference point for HBM counters)")
     ap.add_argument("--counters", action="store_true",
@@ -493,7 +546,7 @@ def main():
                          "GPU but the FP64 RHS steps (and --copy-calibration)")
     args = ap.parse_args()
     if args.counters:
-        args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = True
+        args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = args.no_ccz4 = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the RHS path is single-GPU here); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -624,6 +677,10 @@ def main():
         log(f"large block: fp64 {large['fp64']:.1f} Mpts/s, mixed {large['mixed']:.1f} Mpts/s")
         del lu, lout
 
+    ccz4 = None
+    if not args.no_ccz4:
+        ccz4 = ccz4_section(hev)
+
     if args.copy_calibration:
         import numpy as np
         src = torch.from_numpy(np.ones(1 << 27)).cuda()
@@ -661,7 +718,7 @@ def main():
                                "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
-        "constraint_drift": drift, "large_block": large, "host_dropin": host,
+        "constraint_drift": drift, "large_block": large, "ccz4_large_block": ccz4, "host_dropin": host,
     }
     print(json.dumps(line), flush=True)
 

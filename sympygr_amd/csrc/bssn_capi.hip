@@ -68,6 +68,23 @@ uint64_t rhs_tile_bytes(size_t wsz, int nf)
 }
 uint64_t con_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (NSLOT * wsz + NF * sizeof(double)); }
 
+// The kernels' descriptor of one validated block whose first tile has the
+// global id tile_base.
+BlockDesc block_desc(const sgr_block &b, uint32_t tile_base)
+{
+    BlockDesc d{};
+    d.offset = b.offset;
+    d.bflag = b.bflag;
+    d.tile_base = tile_base;
+    for (int k = 0; k < 3; k++) {
+        d.n[k] = (int32_t)b.sz[k];
+        d.ntile[k] = (int32_t)((b.sz[k] - 6 + T - 1) / T);
+        d.pmin[k] = b.pmin[k];
+        d.ax[k] = axis_const(b.pmin[k], b.pmax[k], b.sz[k]);
+    }
+    return d;
+}
+
 int validate_params(const sgr_params *p)
 {
     if (!p) return fail(SGR_E_INVALID, "params is null");
@@ -237,7 +254,6 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         if (tv && !strcmp(tv, "512")) p->derivs_threads = 512;
         else if (tv && !strcmp(tv, "256")) p->derivs_threads = 256;
     }
-    const int tdim[3] = {T, T, T};
     std::vector<uint32_t> tile_block;
     uint64_t ws_base = 0;
     for (uint32_t b = 0; b < nblocks; b++) {
@@ -246,19 +262,9 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
             delete p;
             return rc;
         }
-        BlockDesc d{};
-        d.offset = blocks[b].offset;
+        BlockDesc d = block_desc(blocks[b], (uint32_t)tile_block.size());
         d.ws_base = ws_base;
-        d.bflag = blocks[b].bflag;
-        d.tile_base = (uint32_t)tile_block.size();
-        uint64_t nt = 1;
-        for (int k = 0; k < 3; k++) {
-            d.n[k] = (int32_t)blocks[b].sz[k];
-            d.ntile[k] = (int32_t)((blocks[b].sz[k] - 6 + tdim[k] - 1) / tdim[k]);
-            d.pmin[k] = blocks[b].pmin[k];
-            d.ax[k] = axis_const(blocks[b].pmin[k], blocks[b].pmax[k], blocks[b].sz[k]);
-            nt *= (uint64_t)d.ntile[k];
-        }
+        const uint64_t nt = (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
         if (tile_block.size() + nt > 0xFFFFFFF0u) {
             delete p;
             return fail(SGR_E_INVALID, "too many tiles");
@@ -390,7 +396,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.tile_block = plan->d_tile_block;
     a.ntiles = plan->ntiles;
     a.ws = plan->d_ws;
-    a.wsT = (double *)((char *)plan->d_ws + (uint64_t)plan->chunk_tiles * TILE_PTS * RS_NSLOT * plan->wsz);
+    a.wsT = (double *)((char *)plan->d_ws + (uint64_t)plan->chunk_tiles * TILE_PTS * rs_nslot(plan->nf) * plan->wsz);
     a.params = device_params(plan->params);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
     a.mixed = plan->params.precision == SGR_PREC_MIXED;
