@@ -195,6 +195,21 @@ int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NU
                   const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
                   const sgr_params *params, int device);
 
+/* The same as sgr_rhs_block on the CURRENT device, asynchronously on `stream`,
+ * allocating nothing: no reference counterpart; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch memory (the block's
+ * descriptor and derivative workspace) is a fixed 32 MiB device array that
+ * each GPU the library runs on holds until the process ends, shared by every
+ * call on that GPU: a block may use at most 32 MiB of it, which holds 87
+ * tiles of 8^3 interior points in FP64 BSSN (a 32^3 interior block has 64;
+ * 139 in mixed precision); larger blocks are refused with SGR_E_INVALID.  Calls that overlap on one GPU (two
+ * streams, or two threads) race on that memory; serialise them.  The entry
+ * point itself calls no hipMalloc, hipFree or allocating routine. */
+int sgr_rhs_block_noalloc(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                          const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                          const sgr_params *params, hipStream_t stream);
+
 /* One block, HOST pointers: stages the block's padded input to the device,
  * computes, and copies the interior RHS back.  This is the backend of the
  * C++ drop-in bssnrhs(...) (bssn/include/rhs.h:33-36). */

@@ -92,6 +92,9 @@ _SIGS = {
     "sgr_rhs_block": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
                                      ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(SgrParams),
                                      ctypes.c_int]),
+    "sgr_rhs_block_noalloc": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
+                                             ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                                             ctypes.POINTER(SgrParams), ctypes.c_void_p]),
     "sgr_bssnrhs_host": (ctypes.c_int, [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(SgrParams),
                                         ctypes.c_int]),
@@ -339,6 +342,22 @@ def rhs_block(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, device=0):
                               (ctypes.c_double * 3)(*pmin),
                               (ctypes.c_double * 3)(*pmax), (ctypes.c_uint32 * 3)(*sz), bflag,
                               ctypes.byref(prm), device))
+    return out
+
+
+def rhs_block_noalloc(out, u, offset, pmin, pmax, sz, bflag, prm: SgrParams, stream=None):
+    """sgr_rhs_block_noalloc: one block on the current device, asynchronous on
+    `stream` (default: torch's current stream), no device allocation (its
+    scratch is the library's fixed per-device array; overlapping calls on
+    one GPU race)."""
+    import torch
+    nf = NUM_VARS_CCZ4 if prm.formulation == FORM["ccz4"] else NUM_VARS
+    need = offset + sz[0] * sz[1] * sz[2]
+    dev = torch.cuda.current_device()
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().sgr_rhs_block_noalloc(_rows(out, nf, need, dev), _rows(u, nf, need, dev), offset,
+                                      (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+                                      (ctypes.c_uint32 * 3)(*sz), bflag, ctypes.byref(prm), ctypes.c_void_p(s)))
     return out
 
 

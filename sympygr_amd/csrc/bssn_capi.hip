@@ -856,6 +856,103 @@ int sgr_bssnrhs_host(double *const out[SGR_NUM_VARS], const double *const in[SGR
     return SGR_OK;
 }
 
+}  // extern "C"
+
+// Allocation-free single-block entry (sgr_rhs_block_noalloc).  Its scratch
+// memory -- the block descriptor, the tile map and the derivative workspace
+// of one block -- is a file-scope device array that every GPU the library
+// runs on holds until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: its device address is taken
+// once per device with hipGetSymbolAddress and only that address is used.
+namespace {
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 32ull << 20;  // per device
+constexpr uint64_t NOALLOC_DESC_OFF = 0;                 // one BlockDesc
+constexpr uint64_t NOALLOC_MAP_OFF = 4096;               // tile map (uint32 per tile)
+constexpr uint64_t NOALLOC_WS_OFF = 65536;               // workspace (256-aligned)
+constexpr uint64_t NOALLOC_MAX_TILES = (NOALLOC_WS_OFF - NOALLOC_MAP_OFF) / sizeof(uint32_t);
+static_assert(sizeof(BlockDesc) <= NOALLOC_MAP_OFF, "block descriptor larger than its scratch slot");
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_m;
+std::vector<void *> g_scratch_ptr;  // device address of `scratch`, per device (null: not yet taken)
+
+// Device address of this device's copy of `scratch` (the device is current).
+int scratch_address(int device, unsigned char **ptr)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_m);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = (unsigned char *)g_scratch_ptr[device];
+    return SGR_OK;
+}
+
+// Writes the block descriptor and the tile map (every tile in block 0) into
+// the scratch memory; the descriptor travels as a kernel argument.
+__global__ void k_setup_block(BlockDesc d, BlockDesc *dst, uint32_t *tile_block, uint32_t ntiles)
+{
+    for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x) tile_block[t] = 0;
+    if (threadIdx.x == 0) *dst = d;
+}
+}  // namespace
+
+extern "C" {
+
+int sgr_rhs_block_noalloc(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
+                          const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
+                          const sgr_params *params, hipStream_t stream)
+{
+    if (!out || !in) return fail(SGR_E_INVALID, "null argument");
+    sgr_block b;
+    int rc = block_checks(pmin, pmax, sz, bflag, params, offset, b);
+    if (rc) return rc;
+    const int nf = params->formulation == SGR_FORM_CCZ4 ? NF_CCZ4 : NF;
+    for (int f = 0; f < nf; f++)
+        if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
+    const BlockDesc d = block_desc(b, 0);
+    const uint64_t ntiles = (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
+    const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
+    const uint64_t need = NOALLOC_WS_OFF + ntiles * rhs_tile_bytes(wsz, nf);
+    if (ntiles > NOALLOC_MAX_TILES || need > NOALLOC_SCRATCH_BYTES) {
+        char msg[200];
+        snprintf(msg, sizeof msg,
+                 "block of %llu tiles needs %.1f MiB of scratch; the allocation-free entry holds %llu MiB",
+                 (unsigned long long)ntiles, need / 1048576.0, (unsigned long long)(NOALLOC_SCRATCH_BYTES >> 20));
+        return fail(SGR_E_INVALID, msg);
+    }
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+    unsigned char *sp = nullptr;
+    rc = scratch_address(device, &sp);
+    if (rc) return rc;
+    BlockDesc *d_desc = (BlockDesc *)(sp + NOALLOC_DESC_OFF);
+    uint32_t *d_map = (uint32_t *)(sp + NOALLOC_MAP_OFF);
+    hipLaunchKernelGGL(k_setup_block, dim3(1), dim3(256), 0, stream, d, d_desc, d_map, (uint32_t)ntiles);
+    HIP_TRY(hipGetLastError(), "k_setup_block launch");
+    LaunchArgs a{};
+    for (int f = 0; f < nf; f++) {
+        a.in.p[f] = in[f];
+        a.out.p[f] = out[f];
+    }
+    a.blocks = d_desc;
+    a.tile_block = d_map;
+    a.ntiles = (uint32_t)ntiles;
+    a.ws = sp + NOALLOC_WS_OFF;
+    a.wsT = (double *)(sp + NOALLOC_WS_OFF + ntiles * TILE_PTS * rs_nslot(nf) * wsz);
+    a.params = device_params(*params);
+    a.upwind = params->adv_mode == SGR_ADV_UPWIND;
+    a.mixed = params->precision == SGR_PREC_MIXED;
+    a.ccz4 = nf == NF_CCZ4;
+    a.rk.stage = -1;
+    const hipError_t e = launch_rhs(a, stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return SGR_OK;
+}
+
 size_t sgr_last_error(char *buf, size_t len)
 {
     if (buf && len) {
