@@ -399,6 +399,36 @@ def ccz4_section(hev, steps=10):
     return res
 
 
+def interior_blocks_leg(bl, u, steps, hev):
+    """The interior-only leg of the 512-block workload: the 216 blocks with
+    no flagged face (bflag 0), run as one plan over the same field arrays
+    (their offsets into them), FP64 and mixed."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    sub = data.BlockList()
+    for b in range(len(bl)):
+        if bl.bflag[b] == 0:
+            sub.offsets.append(bl.offsets[b])
+            sub.sz.append(bl.sz[b])
+            sub.pmin.append(bl.pmin[b])
+            sub.pmax.append(bl.pmax[b])
+            sub.bflag.append(0)
+    sub.total = bl.total
+    pts = sub.interior_points()
+    res = {"blocks": len(sub), "interior_points": pts}
+    out = torch.zeros_like(u)
+    for prec in ("fp64", "mixed"):
+        plan = sgr.Plan(sub, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
+        w, kd, kp = timed_steps(plan, out, u, steps, 2, hev)
+        res[prec] = {"value": pts * steps / w / 1e6, "kernel_ms": {"k_stencil": kd, "k_pointwise": kp},
+                     "roofline": mix_roofline({sub.sz[0][0] - 6: pts}, (kd + kp) * 1e-3, prec)}
+        log(f"interior blocks {prec}: {res[prec]['value']:.1f} Mpts/s")
+        del plan
+    del out
+    return res
+
+
 def host_dropin_rate(bl, u, nsample=16, reps=3):
     """The C++ drop-in path on HOST arrays (sgr_bssnrhs_host, as bssnrhs(...)
     is called by a Dendro driver): one call per block over the first
@@ -543,7 +573,9 @@ def main():
                     help="also run one 1 GiB device-to-device copy (k_hbm_copy, reference point for HBM counters)")
     ap.add_argument("--counters", action="store_true",
                     help="hardware-counter runs: inputs filled on the host and copied over, no work on the "
-                         "GPU but the FP64 RHS steps (and --copy-calibration)")
+                         "GPU but the RHS steps (and --copy-calibration)")
+    ap.add_argument("--counters-prec", choices=("fp64", "mixed"), default="fp64",
+                    help="precision of the RHS steps of a --counters run")
     args = ap.parse_args()
     if args.counters:
         args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = args.no_ccz4 = True
@@ -572,11 +604,13 @@ def main():
     pts = bl.interior_points()
     hev = HipEvents()
 
-    plan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64", ko_sigma=0.1))
-    log(f"fp64 plan: workspace {plan.info().workspace_bytes / 2**30:.1f} GiB; warmup {args.warmup}, steps {args.steps}")
+    prec0 = args.counters_prec if args.counters else "fp64"
+    plan = sgr.Plan(bl, sgr.params(adv="upwind", precision=prec0, ko_sigma=0.1))
+    log(f"{prec0} plan: workspace {plan.info().workspace_bytes / 2**30:.1f} GiB; warmup {args.warmup}, "
+        f"steps {args.steps}")
     wall, kd, kp = timed_steps(plan, out, u, args.steps, args.warmup, hev)
     value = pts * args.steps / wall / 1e6
-    log(f"fp64: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_stencil {kd:.3f} ms, "
+    log(f"{prec0}: {value:.1f} Mpts/s, {1e3 * wall / args.steps:.3f} ms/step, k_stencil {kd:.3f} ms, "
         f"k_pointwise {kp:.3f} ms")
     ref_out = out if args.counters else out.clone()
     del plan
@@ -598,6 +632,10 @@ def main():
                  "per_field_normwise_rel_err": [float(f"{x:.3e}") for x in err.tolist()],
                  "per_field_rms_rel_err": [float(f"{x:.3e}") for x in rms.tolist()]}
         del mplan, mout
+
+    interior = None
+    if not args.counters:
+        interior = interior_blocks_leg(bl, u, args.steps, hev)
 
     rhs_ms = kd + kp
     abytes = alg_bytes_per_point(n) * pts
@@ -716,9 +754,10 @@ def main():
         "data": "synthetic: Brill-Lindquist two-puncture data, analytic ghosts",
         "config": {"workload": "512 blocks (8^3 tiling of [-8,8]^3) of 32^3 interior + 3 ghost (38^3), "
                                "FP64, 6th-order stencils, upwinded advection, KO sigma 0.1",
-                   "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": "fp64"},
+                   "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": prec0},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
-        "constraint_drift": drift, "large_block": large, "ccz4_large_block": ccz4, "host_dropin": host,
+        "constraint_drift": drift, "large_block": large, "ccz4_large_block": ccz4, "interior_blocks": interior,
+        "host_dropin": host,
     }
     print(json.dumps(line), flush=True)
 

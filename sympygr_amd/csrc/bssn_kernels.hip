@@ -105,7 +105,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
     const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
-    const AxisConst a0 = B.ax[0], a1 = B.ax[1], a2 = B.ax[2];
+    const AxisConst &a0 = B.ax[0], &a1 = B.ax[1], &a2 = B.ax[2];
     // does this tile contain a point of a flagged face? (uniform)
     const bool face_tile = (lo0 && tc.i0 == GW) || (hi0 && tc.i0 + T >= n0 - GW) || (lo1 && tc.j0 == GW) ||
                            (hi1 && tc.j0 + T >= n1 - GW) || (lo2 && tc.k0 == GW) || (hi2 && tc.k0 + T >= n2 - GW);
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
     const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
-    const AxisConst a0 = B.ax[0], a1 = B.ax[1], a2 = B.ax[2];
+    const AxisConst &a0 = B.ax[0], &a1 = B.ax[1], &a2 = B.ax[2];
     // face closures apply to the three points next to a flagged face: only
     // tiles that reach them take the per-point closure branches (uniform)
     const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + T - 1 >= n0 - GW - 3;
