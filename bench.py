@@ -671,7 +671,7 @@ def main():
         "bound": "fp64_vector" if bnd["binding"] == "valu" else "hbm",
         "achieved": achieved_tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved_tfs / FP64_PEAK_TFS,
         "traffic": traffic, "traffic_detail": traffic_detail,
-        "kernel": "RHS pass = k_stencil + k_pointwise_g1 + k_pointwise_g2 (HIP events around each launch on "
+        "kernel": "RHS pass = k_stencil + k_pointwise (HIP events around each launch on "
                   "the launch stream)",
         "flops_per_point": FLOPS_UPWIND, "bound_ps_per_point": bnd,
         "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,

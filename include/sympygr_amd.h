@@ -131,10 +131,14 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
                    hipStream_t stream, hipEvent_t events[4]);
 
 /* Hamiltonian constraint (out[0]) and momentum constraints (out[1..3]) at
- * every interior point of the plan's blocks, in FP64 (GR/constraints.py:
- * 161-173, same derivative operators as the RHS).  Uses the plan's
- * workspace: do not overlap with sgr_rhs on the same plan.  The reference's
- * generated counterpart (bssn/CodeGen/physcon.cpp) does not compile. */
+ * every interior point of the plan's blocks (GR/constraints.py:161-173, same
+ * derivative operators as the RHS), computed and stored in FP64.  The
+ * derivatives are staged in the plan's workspace type: FP64 for an FP64
+ * plan; FP32 for a mixed-precision plan, whose constraints therefore carry
+ * FP32 rounding of the derivatives (about 1e-7 relative) -- use an FP64 plan
+ * for FP64 constraints.  Uses the plan's workspace: do not overlap with
+ * sgr_rhs on the same plan.  The reference's generated counterpart
+ * (bssn/CodeGen/physcon.cpp) does not compile. */
 int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream);
 
 /* Ghost refresh (the unzip step between RK stages), in place.  Every block

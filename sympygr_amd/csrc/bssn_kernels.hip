@@ -1,32 +1,30 @@
-// BSSN RHS kernels for gfx950 (CDNA4).
+// BSSN (and CCZ4) RHS kernels for gfx950 (CDNA4).
 //
-// Two passes over 8x8x8 interior tiles of every block (tile-major workspace):
+// The RHS is two passes over 8x8x8 interior tiles of every block, with a
+// tile-major derivative workspace between them:
 //
-//   k_derivs     one workgroup per tile.  Field by field, the 14^3 halo tile
-//                is staged in LDS; every thread evaluates the first, second,
-//                advective and Kreiss-Oliger stencils of its point (512
-//                threads; or of two points, 256 threads) from LDS.  The inner first derivatives of the mixed terms
-//                (D_x u on the tile's y/z halo, D_y u on its z halo) are
-//                staged in LDS as well, so D_y D_x, D_z D_x and D_z D_y are
-//                the same compositions as the reference's sweeps
-//                (bssn/scripts/bssnrhs_derivs.h:7-9).  Results go to the
-//                derivative workspace (double, or float in mixed mode) and
-//                the KO sums (always double).  Two contractions are done
-//                here, where all their factors are at hand, so the workspace
-//                holds one value instead of three or six: the advection
-//                beta^d ad_d f of every field and igt^{lm} d_l d_m gt_ij.
+//   k_stencil    one 512-thread workgroup per tile.  Field by field, the 14^3
+//                halo tile is staged in LDS (the next field's tile in flight
+//                meanwhile) and every thread evaluates its point's first,
+//                advective, Kreiss-Oliger and second-derivative stencils;
+//                the inner derivatives of the mixed terms are staged in LDS
+//                so that D_y D_x, D_z D_x and D_z D_y are the reference's
+//                compositions (bssn/scripts/bssnrhs_derivs.h:7-9).  It stores
+//                the derivatives the algebra reads (double, or float in mixed
+//                mode; the shift and gt second derivatives already contracted
+//                with the inverse metric) and each output's tail in double:
+//                c_f beta.ad f + sigma KO f, or on a flagged-face point the
+//                Sommerfeld value plus sigma KO.
 //
-//   k_pointwise_g{1,2}
-//                one thread per interior point, one kernel per output group
-//                (alpha/beta/chi/gt; At/K/Gt/B): reads its fields and
-//                staged derivatives, runs that group's part of the generated
-//                algebra in R (double or float), applies the Sommerfeld
-//                condition on flagged faces, adds sigma * KO in double and
-//                writes the RHS.  Splitting by group keeps each kernel's
-//                register file small enough for several waves per SIMD.
+//   k_pointwise  one thread per interior point: reads its fields, staged
+//                derivatives and tails, runs the generated algebra of all
+//                outputs in R (double or float), adds each tail in double and
+//                writes the RHS or the fused RK4 stage update.
 //
-// Both kernels read the block table and a tile->block map, so one launch
-// covers any mix of block sizes.
+// Both read the block table and a tile->block map, so one launch covers any
+// mix of block sizes.  k_derivs + k_constraints are the constraint pass (the
+// stencil kernel of the earlier layout, which stages every first derivative);
+// k_ghost_sep / k_ghost_copy refresh ghost nodes from the owning blocks.
 #include <hip/hip_ext.h>
 
 #include "bssn_kernels.h"
@@ -681,27 +679,18 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
     }
 }
 
+// the BSSN system (gen/bssn_algebra_pw.inc, sympygr_amd/codegen/bssn_eqs.py)
 #define SGR_PW_NFLD NF
-#define SGR_PW_NAME k_pointwise_g1
-#define SGR_PW_ALG "gen/bssn_algebra_g1.inc"
-#include "pointwise_body.h"
-#undef SGR_PW_NAME
-#undef SGR_PW_ALG
-#define SGR_PW_NAME k_pointwise_g2
-#define SGR_PW_ALG "gen/bssn_algebra_g2.inc"
+#define SGR_PW_NAME k_pointwise
+#define SGR_PW_ALG "gen/bssn_algebra_pw.inc"
 #include "pointwise_body.h"
 #undef SGR_PW_NAME
 #undef SGR_PW_ALG
 #undef SGR_PW_NFLD
-// the CCZ4 system (gen/ccz4_algebra_g*.inc, sympygr_amd/codegen/ccz4_eqns.py)
+// the CCZ4 system (gen/ccz4_algebra_pw.inc, sympygr_amd/codegen/ccz4_eqns.py)
 #define SGR_PW_NFLD NF_CCZ4
-#define SGR_PW_NAME k_ccz4_pointwise_g1
-#define SGR_PW_ALG "gen/ccz4_algebra_g1.inc"
-#include "pointwise_body.h"
-#undef SGR_PW_NAME
-#undef SGR_PW_ALG
-#define SGR_PW_NAME k_ccz4_pointwise_g2
-#define SGR_PW_ALG "gen/ccz4_algebra_g2.inc"
+#define SGR_PW_NAME k_ccz4_pointwise
+#define SGR_PW_ALG "gen/ccz4_algebra_pw.inc"
 #include "pointwise_body.h"
 #undef SGR_PW_NAME
 #undef SGR_PW_ALG
@@ -720,19 +709,15 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
                               a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipExtLaunchKernelGGL((k_ccz4_pointwise_g1<R, W, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
+        hipExtLaunchKernelGGL((k_ccz4_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                               a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
-        hipExtLaunchKernelGGL((k_ccz4_pointwise_g2<R, W, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out,
-                              a.blocks, a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
         return hipGetLastError();
     }
     hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NF>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
                           a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipExtLaunchKernelGGL((k_pointwise_g1<R, W, RK>), g, b, 0, s, a.ev[2], nullptr, 0, a.in, a.out, a.blocks,
-                          a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
-    hipExtLaunchKernelGGL((k_pointwise_g2<R, W, RK>), g, b, 0, s, nullptr, a.ev[3], 0, a.in, a.out, a.blocks,
+    hipExtLaunchKernelGGL((k_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
     return hipGetLastError();
 }

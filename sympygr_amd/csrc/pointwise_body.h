@@ -1,6 +1,7 @@
-// Body of one output-group pointwise kernel; included once per group with
-// SGR_PW_NAME (kernel name) and SGR_PW_ALG (generated algebra file) defined.
-// The group's pointwise algebra at every interior point its tile owns.  One
+// Body of the pointwise kernel; included once per system with SGR_PW_NFLD
+// (field count), SGR_PW_NAME (kernel name) and SGR_PW_ALG (generated algebra
+// file) defined.  The pointwise algebra at every interior point its tile
+// owns.  One
 // thread per point: reads the point's field values, the derivatives the
 // stencil pass staged and the per-output tails (advection plus sigma times
 // Kreiss-Oliger, double), runs the algebra in R (double, or float in mixed
