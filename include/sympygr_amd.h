@@ -191,10 +191,11 @@ int sgr_graph_launch(sgr_graph *graph, hipStream_t stream);
 int sgr_graph_destroy(sgr_graph *graph);
 
 /* One block, DEVICE pointers, synchronous: bssnrhs (bssn/include/rhs.h:33-36)
- * with the arrays already on the device.  The per-block entries share one
- * process-wide context (a one-block plan and staging buffers, rebuilt only
- * when the device, block size, advection mode or precision changes; held
- * until the process ends); concurrent calls are serialised. */
+ * with the arrays already on the device.  The per-block entries keep one
+ * context per calling thread (a one-block plan, a stream and staging
+ * buffers, rebuilt only when the device, block size, advection mode,
+ * precision or formulation changes; held until the process ends), so calls
+ * from different host threads proceed concurrently. */
 int sgr_rhs_block(double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS], uint64_t offset,
                   const double pmin[3], const double pmax[3], const uint32_t sz[3], uint32_t bflag,
                   const sgr_params *params, int device);
