@@ -642,8 +642,8 @@ def main():
     achieved_gbs = abytes / (rhs_ms * 1e-3) / 1e9
     achieved_tfs = FLOPS_UPWIND * pts / (rhs_ms * 1e-3) / 1e12
     dominant = "k_pointwise" if kp >= kd else "k_stencil"
-    # per kernel: k_stencil does the stencils and the KO sums (FP64) and
-    # reads the fields over the padded blocks; the pointwise kernels run the
+    # per kernel: k_stencil does the stencils and the KO terms (FP64) and
+    # reads the fields over the padded blocks; the pointwise kernel runs the
     # algebra
     st_flops = FLOPS_STENCIL + FLOPS_KO_ADD
     st_tfs = st_flops * pts / (kd * 1e-3) / 1e12

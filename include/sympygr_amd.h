@@ -123,7 +123,7 @@ int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const
             hipStream_t stream);
 
 /* Same as sgr_rhs, and records events[0..1] around the derivative kernel and
- * events[2..3] around the pointwise kernels on `stream` (hipEventElapsedTime
+ * events[2..3] around the pointwise kernel on `stream` (hipEventElapsedTime
  * then gives each pass's duration).  For a plan that runs in several
  * workspace chunks, events[0..1] bracket the whole call and events[2..3] are
  * recorded at its end.  No reference counterpart: it exists for measurement. */

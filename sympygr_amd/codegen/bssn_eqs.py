@@ -124,9 +124,9 @@ class Stages:
         self.stages.append((name, list(pairs)))
 
 
-# Fields whose advection the GPU pointwise kernels do not read: the stencil
+# Fields whose advection the GPU pointwise kernel does not read: the stencil
 # pass folds it, with the Kreiss-Oliger term, into a per-output tail that the
-# kernels add after the algebra (every field).
+# kernel adds after the algebra (every field).
 TAIL_FIELDS = set(FIELDS)
 
 

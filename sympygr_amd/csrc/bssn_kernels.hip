@@ -438,7 +438,7 @@ __device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, i
 // chi, the shift and gt also the second derivatives, the mixed ones as
 // compositions D_y D_x, D_z D_x, D_z D_y through the LDS-staged inner
 // derivatives (bssn/scripts/bssnrhs_derivs.h:7-9).  It stages for the
-// pointwise kernels what the algebra reads (the RS_* slots) and each
+// pointwise kernel what the algebra reads (the RS_* slots) and each
 // output's tail (its advection and Kreiss-Oliger terms, in double); on a
 // flagged-face point the tail is the whole output, the Sommerfeld value
 // (bssn/src/rhs.cpp:833-977) plus sigma KO.
