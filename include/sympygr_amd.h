@@ -98,7 +98,9 @@ typedef struct sgr_plan sgr_plan;
 
 typedef struct {
     uint64_t blocks;
-    uint64_t tiles;              /* 8^3 work tiles */
+    uint64_t tiles;              /* RHS work tiles: 8^3, or the shape tuned for the block's
+                                    size (the library's tile table; SGR_TILE_SHAPE=tx,ty,tz
+                                    forces one shape on every block) */
     uint64_t interior_points;
     uint64_t workspace_bytes;    /* device scratch held by the plan */
     uint32_t derivative_slots;   /* derivative values staged per point */

@@ -68,22 +68,60 @@ uint64_t rhs_tile_bytes(size_t wsz, int nf)
 }
 uint64_t con_tile_bytes(size_t wsz) { return (uint64_t)TILE_PTS * (NSLOT * wsz + NF * sizeof(double)); }
 
-// The kernels' descriptor of one validated block whose first tile has the
-// global id tile_base.
-BlockDesc block_desc(const sgr_block &b, uint32_t tile_base)
+// RHS tile shapes tuned per interior size (gen/tile_table.inc, written by
+// `python -m sympygr_amd.autotune`); a size not in the table uses 8^3.
+struct TileShapeEntry {
+    uint16_t n[3];  // interior points per axis
+    uint8_t t[3];   // tile extents
+};
+const TileShapeEntry kTileTable[] = {
+#include "gen/tile_table.inc"
+    {{0, 0, 0}, {T, T, T}},
+};
+
+// The RHS tile shape of a block: SGR_TILE_SHAPE=tx,ty,tz (every block; for
+// tuning and tests), else the table, else 8^3.
+int tile_shape(const sgr_block &b, int td[3])
+{
+    td[0] = td[1] = td[2] = T;
+    if (const char *sv = getenv("SGR_TILE_SHAPE")) {
+        if (sscanf(sv, "%d,%d,%d", &td[0], &td[1], &td[2]) != 3 || td[0] < 2 || td[1] < 2 || td[2] < 2 ||
+            !tile_shape_ok(td[0], td[1], td[2]))
+            return fail(SGR_E_INVALID, std::string("SGR_TILE_SHAPE=") + sv +
+                                           " is not a tile shape (tx,ty,tz >= 2, at most 512 points, within the "
+                                           "stencil kernel's LDS arrays)");
+        return SGR_OK;
+    }
+    for (const TileShapeEntry &e : kTileTable) {
+        if (e.n[0] == b.sz[0] - 6 && e.n[1] == b.sz[1] - 6 && e.n[2] == b.sz[2] - 6) {
+            for (int k = 0; k < 3; k++) td[k] = e.t[k];
+            break;
+        }
+    }
+    return SGR_OK;
+}
+
+// The kernels' descriptor of one validated block with RHS tile extents td,
+// whose first RHS tile has the global id tile_base and first constraint-pass
+// tile the id tile_base8.
+BlockDesc block_desc(const sgr_block &b, const int td[3], uint32_t tile_base, uint32_t tile_base8)
 {
     BlockDesc d{};
     d.offset = b.offset;
     d.bflag = b.bflag;
     d.tile_base = tile_base;
+    d.tile_base8 = tile_base8;
     for (int k = 0; k < 3; k++) {
         d.n[k] = (int32_t)b.sz[k];
-        d.ntile[k] = (int32_t)((b.sz[k] - 6 + T - 1) / T);
+        d.tdim[k] = td[k];
+        d.ntile[k] = (int32_t)((b.sz[k] - 6 + td[k] - 1) / td[k]);
+        d.ntile8[k] = (int32_t)((b.sz[k] - 6 + T - 1) / T);
         d.pmin[k] = b.pmin[k];
         d.ax[k] = axis_const(b.pmin[k], b.pmax[k], b.sz[k]);
     }
     return d;
 }
+bool is_cube8(const BlockDesc &d) { return d.tdim[0] == T && d.tdim[1] == T && d.tdim[2] == T; }
 
 int validate_params(const sgr_params *p)
 {
@@ -173,8 +211,11 @@ struct sgr_plan {
     sgr_params params{};
     std::vector<BlockDesc> hblocks;
     BlockDesc *d_blocks = nullptr;
-    uint32_t *d_tile_block = nullptr;
+    uint32_t *d_tile_block = nullptr;   // RHS tiles: the 8^3-shaped blocks' tiles first
+    uint32_t *d_tile_block8 = nullptr;  // constraint-pass tiles (8^3), block order
     uint32_t ntiles = 0;
+    uint32_t nfast = 0;                 // RHS tiles of 8^3-shaped blocks (tile ids 0 .. nfast-1)
+    uint32_t ntiles8 = 0;
     // One device workspace, tile-major, shared by the two passes that use it:
     //   RHS:         chunk_tiles tiles of rs_nslot(nf) W values per point,
     //                then their nf double tails per point
@@ -202,6 +243,7 @@ static void plan_free(sgr_plan *p)
     hipSetDevice(p->device);
     hipFree(p->d_blocks);
     hipFree(p->d_tile_block);
+    hipFree(p->d_tile_block8);
     hipFree(p->d_ws);
     hipFree(p->d_flag);
     hipFree(p->d_ghost);
@@ -254,27 +296,48 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         if (tv && !strcmp(tv, "512")) p->derivs_threads = 512;
         else if (tv && !strcmp(tv, "256")) p->derivs_threads = 256;
     }
-    std::vector<uint32_t> tile_block;
-    uint64_t ws_base = 0;
+    // RHS tiles of the 8^3-shaped blocks first, then those of other shapes
+    // (two stencil kernels, each over a contiguous tile range); constraint
+    // tiles (8^3) in block order
+    std::vector<uint32_t> tile_block, tile_block8;
+    std::vector<int> shape(3 * (size_t)nblocks);
     for (uint32_t b = 0; b < nblocks; b++) {
         rc = validate_block(blocks[b], b);
+        if (!rc) rc = tile_shape(blocks[b], &shape[3 * b]);
         if (rc) {
             delete p;
             return rc;
         }
-        BlockDesc d = block_desc(blocks[b], (uint32_t)tile_block.size());
-        d.ws_base = ws_base;
-        const uint64_t nt = (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
-        if (tile_block.size() + nt > 0xFFFFFFF0u) {
+        p->hblocks.push_back(block_desc(blocks[b], &shape[3 * b], 0, 0));
+        p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t b = 0; b < nblocks; b++) {
+            BlockDesc &d = p->hblocks[b];
+            if (is_cube8(d) != (pass == 0)) continue;
+            const uint64_t nt = (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
+            if (tile_block.size() + nt > 0xFFFFFFF0u) {
+                delete p;
+                return fail(SGR_E_INVALID, "too many tiles");
+            }
+            d.tile_base = (uint32_t)tile_block.size();
+            d.ws_base = (uint64_t)d.tile_base * TILE_PTS;
+            tile_block.insert(tile_block.end(), nt, b);
+        }
+        if (pass == 0) p->nfast = (uint32_t)tile_block.size();
+    }
+    for (uint32_t b = 0; b < nblocks; b++) {
+        BlockDesc &d = p->hblocks[b];
+        const uint64_t nt = (uint64_t)d.ntile8[0] * d.ntile8[1] * d.ntile8[2];
+        if (tile_block8.size() + nt > 0xFFFFFFF0u) {
             delete p;
             return fail(SGR_E_INVALID, "too many tiles");
         }
-        for (uint64_t t = 0; t < nt; t++) tile_block.push_back(b);
-        ws_base += nt * TILE_PTS;
-        p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
-        p->hblocks.push_back(d);
+        d.tile_base8 = (uint32_t)tile_block8.size();
+        tile_block8.insert(tile_block8.end(), nt, b);
     }
     p->ntiles = (uint32_t)tile_block.size();
+    p->ntiles8 = (uint32_t)tile_block8.size();
     // The workspace holds at most `chunk_tiles` tiles of the RHS pass (budget
     // SGR_WS_GIB, default 40 GiB); larger plans run as several launches over
     // equal tile ranges.  The constraint pass stages more per tile and runs
@@ -293,13 +356,14 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         const uint64_t nchunks = nt ? (nt + cap - 1) / cap : 1;
         p->chunk_tiles = (uint32_t)((nt + nchunks - 1) / nchunks);
         p->ws_bytes = nt ? std::max<uint64_t>(p->chunk_tiles * rhs_tile, con_tile) : 0;
-        p->cchunk_tiles = (uint32_t)std::min<uint64_t>(nt, p->ws_bytes / con_tile);
+        p->cchunk_tiles = (uint32_t)std::min<uint64_t>(tile_block8.size(), p->ws_bytes / con_tile);
     }
     // optional cap on the plan's device memory (SGR_DEVICE_MEM_LIMIT_GIB):
     // a plan that would need more is refused before anything is allocated
     {
         const char *lv = getenv("SGR_DEVICE_MEM_LIMIT_GIB");
-        const uint64_t need = p->ws_bytes + nblocks * sizeof(BlockDesc) + tile_block.size() * sizeof(uint32_t);
+        const uint64_t need = p->ws_bytes + nblocks * sizeof(BlockDesc) +
+                              (tile_block.size() + tile_block8.size()) * sizeof(uint32_t);
         if (lv && (double)need > atof(lv) * 1073741824.0) {
             char msg[160];
             snprintf(msg, sizeof msg, "plan needs %.3f GiB of device memory, over SGR_DEVICE_MEM_LIMIT_GIB=%s",
@@ -345,6 +409,11 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         if ((e = hipMemcpy(p->d_tile_block, tile_block.data(), tile_block.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice)) != hipSuccess)
             return bail(e, "hipMemcpy(tiles)");
+        if ((e = dev_alloc((void **)&p->d_tile_block8, tile_block8.size() * sizeof(uint32_t))) != hipSuccess)
+            return bail(e, "hipMalloc(constraint tiles)");
+        if ((e = hipMemcpy(p->d_tile_block8, tile_block8.data(), tile_block8.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return bail(e, "hipMemcpy(constraint tiles)");
         if ((e = dev_alloc(&p->d_ws, p->ws_bytes)) != hipSuccess) return bail(e, "hipMalloc(workspace)");
     }
     if ((e = dev_alloc((void **)&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
@@ -395,6 +464,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     a.blocks = plan->d_blocks;
     a.tile_block = plan->d_tile_block;
     a.ntiles = plan->ntiles;
+    a.nfast = plan->nfast;
     a.ws = plan->d_ws;
     a.wsT = (double *)((char *)plan->d_ws + (uint64_t)plan->chunk_tiles * TILE_PTS * rs_nslot(plan->nf) * plan->wsz);
     a.params = device_params(plan->params);
@@ -419,6 +489,7 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
         for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->chunk_tiles) {
             a.tile0 = t0;
             a.ntiles = plan->ntiles - t0 < plan->chunk_tiles ? plan->ntiles - t0 : plan->chunk_tiles;
+            a.nfast = plan->nfast <= t0 ? 0 : std::min(plan->nfast - t0, a.ntiles);
             e = launch_rhs(a, stream);
         }
         if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
@@ -456,8 +527,8 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
         co.p[c] = out[c];
     }
     a.blocks = plan->d_blocks;
-    a.tile_block = plan->d_tile_block;
-    a.ntiles = plan->ntiles;
+    a.tile_block = plan->d_tile_block8;
+    a.ntiles = plan->ntiles8;
     a.ws = plan->d_ws;
     a.wsK = (double *)((char *)plan->d_ws + (uint64_t)plan->cchunk_tiles * TILE_PTS * NSLOT * plan->wsz);
     a.upwind = plan->params.adv_mode == SGR_ADV_UPWIND;
@@ -467,9 +538,9 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
     hipError_t e = hipSuccess;
-    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->cchunk_tiles) {
+    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles8; t0 += plan->cchunk_tiles) {
         a.tile0 = t0;
-        a.ntiles = plan->ntiles - t0 < plan->cchunk_tiles ? plan->ntiles - t0 : plan->cchunk_tiles;
+        a.ntiles = plan->ntiles8 - t0 < plan->cchunk_tiles ? plan->ntiles8 - t0 : plan->cchunk_tiles;
         e = launch_constraints(a, co, stream);
     }
     if (cur != plan->device) hipSetDevice(cur);
@@ -913,7 +984,10 @@ int sgr_rhs_block_noalloc(double *const out[SGR_NUM_VARS], const double *const i
     const int nf = params->formulation == SGR_FORM_CCZ4 ? NF_CCZ4 : NF;
     for (int f = 0; f < nf; f++)
         if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");
-    const BlockDesc d = block_desc(b, 0);
+    int td[3];
+    rc = tile_shape(b, td);
+    if (rc) return rc;
+    const BlockDesc d = block_desc(b, td, 0, 0);
     const uint64_t ntiles = (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
     const size_t wsz = params->precision == SGR_PREC_MIXED ? sizeof(float) : sizeof(double);
     const uint64_t need = NOALLOC_WS_OFF + ntiles * rhs_tile_bytes(wsz, nf);
@@ -941,6 +1015,7 @@ int sgr_rhs_block_noalloc(double *const out[SGR_NUM_VARS], const double *const i
     a.blocks = d_desc;
     a.tile_block = d_map;
     a.ntiles = (uint32_t)ntiles;
+    a.nfast = is_cube8(d) ? (uint32_t)ntiles : 0;
     a.ws = sp + NOALLOC_WS_OFF;
     a.wsT = (double *)(sp + NOALLOC_WS_OFF + ntiles * TILE_PTS * rs_nslot(nf) * wsz);
     a.params = device_params(*params);

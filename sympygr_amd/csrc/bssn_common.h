@@ -182,17 +182,35 @@ struct AxisConst {
     double smr3, smr2, smr1;  // 59/48*64*h, 43/48*64*h, 49/48*64*h   derivs.cpp:921-923
 };
 
-// One padded block as the kernels see it.
+// One padded block as the kernels see it.  The RHS pass tiles a block with
+// tiles of tdim[0] x tdim[1] x tdim[2] interior points (at most TILE_PTS;
+// 8^3, or the shape tuned for the block's size); the constraint pass always
+// uses 8^3 tiles.
 struct BlockDesc {
     uint64_t offset;     // start of the block in every field array
     uint64_t ws_base;    // first workspace point of the block (tile-major)
     int32_t n[3];        // padded sizes
-    int32_t ntile[3];    // tiles per axis = ceil((n-6)/T)
+    int32_t ntile[3];    // RHS tiles per axis = ceil((n-6)/tdim)
+    int32_t tdim[3];     // RHS tile extents
+    int32_t ntile8[3];   // constraint-pass tiles per axis = ceil((n-6)/8)
     uint32_t bflag;
-    uint32_t tile_base;  // first global tile id of this block
+    uint32_t tile_base;  // first RHS tile id of this block
+    uint32_t tile_base8; // first constraint-pass tile id of this block
     double pmin[3];
     AxisConst ax[3];
 };
+
+// Limits of a general RHS tile shape (k_stencil_gen's LDS arrays): halo tile
+// (tx+6)(ty+6)(tz+6), D_x stage tx(ty+6)(tz+6), D_y stage tx ty (tz+6).
+constexpr int GEN_U_CAP = 3456;
+constexpr int GEN_DX_CAP = 3200;
+constexpr int GEN_DY_CAP = 1600;
+__host__ __device__ constexpr bool tile_shape_ok(int tx, int ty, int tz)
+{
+    return tx >= 1 && ty >= 1 && tz >= 1 && tx * ty * tz <= TILE_PTS &&
+           (tx + 6) * (ty + 6) * (tz + 6) <= GEN_U_CAP && tx * (ty + 6) * (tz + 6) <= GEN_DX_CAP &&
+           tx * ty * (tz + 6) <= GEN_DY_CAP;
+}
 
 // Physics parameters (bssn/src/parameters.cpp:99-109).
 struct Params {

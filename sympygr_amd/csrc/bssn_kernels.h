@@ -26,8 +26,9 @@ struct LaunchArgs {
     const uint32_t *tile_block;  // device, block id of every tile
     uint32_t ntiles;             // tiles of this launch (one workspace chunk)
     uint32_t tile0;              // first tile of this launch
+    uint32_t nfast;              // RHS: the first nfast tiles of the launch have the 8^3 shape, the rest others
     void *ws;                    // derivative workspace (double or float), tile-major
-    double *wsT;                 // RHS pass: tails of the pointwise outputs (RS_NTAIL per point)
+    double *wsT;                 // RHS pass: tails of the pointwise outputs (one per field and point)
     double *wsK;                 // constraint pass: KO sums (24 per point)
     Params params;
     bool upwind;

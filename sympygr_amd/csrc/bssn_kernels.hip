@@ -42,31 +42,41 @@ struct TileCoord {
     int oi, oj, ok;   // first coordinate the tile owns along each axis
 };
 
-__device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t tile)
+// Tile `lt` of a block tiled with tiles of extents td[] (nt[] tiles per axis).
+// The last tile along an axis is shifted back to end at the interior's end,
+// so it overlaps its neighbour.  Every tile evaluates derivatives at all its
+// points (each tile has its own workspace), but an output point belongs to
+// exactly one tile: the one whose unshifted range [GW + td t, GW + td (t + 1))
+// holds it.  Stores skip the points a shifted tile shares with the previous
+// one, so read-modify-write outputs (the RK4 accumulator) are updated once.
+__device__ __forceinline__ TileCoord tile_at(const BlockDesc &B, int lt, const int32_t nt[3], int d0, int d1, int d2)
 {
     TileCoord c;
-    c.lt = (int)(tile - B.tile_base);
-    const int tx = c.lt % B.ntile[0];
-    const int ty = (c.lt / B.ntile[0]) % B.ntile[1];
-    const int tz = c.lt / (B.ntile[0] * B.ntile[1]);
-    // The last tile along an axis is shifted back to end at the interior's
-    // end, so it overlaps its neighbour.  Every tile evaluates derivatives at
-    // all its points (each tile has its own workspace), but an output point
-    // belongs to exactly one tile: the one whose unshifted range
-    // [GW + T t, GW + T (t + 1)) holds it.  Stores skip the points a shifted
-    // tile shares with the previous one, so read-modify-write outputs (the RK4
-    // accumulator) are updated once.
-    auto origin = [](int tt, int n) {
-        const int o = GW + T * tt, last = n - GW - T;
+    c.lt = lt;
+    const int tx = lt % nt[0];
+    const int ty = (lt / nt[0]) % nt[1];
+    const int tz = lt / (nt[0] * nt[1]);
+    auto origin = [](int tt, int n, int td) {
+        const int o = GW + td * tt, last = n - GW - td;
         return o < last ? o : (last > GW ? last : GW);
     };
-    c.i0 = origin(tx, B.n[0]);
-    c.j0 = origin(ty, B.n[1]);
-    c.k0 = origin(tz, B.n[2]);
-    c.oi = GW + T * tx;
-    c.oj = GW + T * ty;
-    c.ok = GW + T * tz;
+    c.i0 = origin(tx, B.n[0], d0);
+    c.j0 = origin(ty, B.n[1], d1);
+    c.k0 = origin(tz, B.n[2], d2);
+    c.oi = GW + d0 * tx;
+    c.oj = GW + d1 * ty;
+    c.ok = GW + d2 * tz;
     return c;
+}
+
+// RHS tiling (the block's tile shape) and the constraint pass's 8^3 tiling.
+__device__ __forceinline__ TileCoord tile_coord(const BlockDesc &B, uint32_t tile)
+{
+    return tile_at(B, (int)(tile - B.tile_base), B.ntile, B.tdim[0], B.tdim[1], B.tdim[2]);
+}
+__device__ __forceinline__ TileCoord tile_coord8(const BlockDesc &B, uint32_t tile)
+{
+    return tile_at(B, (int)(tile - B.tile_base8), B.ntile8, T, T, T);
 }
 
 // Fields whose first derivatives the pointwise algebra never reads (B, At):
@@ -77,6 +87,15 @@ __host__ __device__ constexpr bool d1_only_for_faces(int f) { return (f >= 9 && 
 // Linear halo element e = ex + 14 (ey + 14 ez) -> its LDS index with rows of
 // 24 doubles: ex + 24 (ey + 14 ez) = e + 10 (e / 14).
 __device__ __forceinline__ int lds_halo(int e) { return e + 10 * (e / TH); }
+
+// Division of a non-negative n < 2^16 by a run-time divisor 2 <= d <= 256
+// through its multiplicative inverse m = ceil(2^32 / d): floor(n / d) =
+// mulhi(n, m) exactly, since n (m d - 2^32) < n d < 2^32 / d.
+struct Div {
+    uint32_t m;
+};
+__device__ __forceinline__ Div make_div(int d) { return Div{0xFFFFFFFFu / (uint32_t)d + 1u}; }
+__device__ __forceinline__ int div_q(int n, const Div &v) { return (int)__umulhi((uint32_t)n, v.m); }
 
 // NT threads per tile: 256 (two points per thread) or 512 (one point each).
 template <typename W, bool UPWIND, int NT>
@@ -99,7 +118,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 3 : 4) void k_derivs(FieldPtrs in, 
     const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
     const uint32_t tile = tile0 + ct;
     const BlockDesc &B = blocks[tile_block[tile]];
-    const TileCoord tc = tile_coord(B, tile);
+    const TileCoord tc = tile_coord8(B, tile);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
     const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
@@ -431,7 +450,11 @@ __device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, i
     }
 }
 
-// k_stencil: one 512-thread workgroup per 8^3 tile, one point per thread.
+// k_stencil: one 512-thread workgroup per tile, one point per thread.  GEN
+// false: 8^3 tiles with conflict-free padded LDS rows and constant
+// indexing; GEN true: any tile shape that tile_shape_ok admits (the block's
+// tdim, read at run time; threads beyond the tile's points only help load
+// and stage), unpadded LDS.
 // Field by field, the 14^3 halo tile is staged in LDS (the next field's tile
 // in flight from HBM meanwhile) and every thread evaluates its point's
 // stencils: first, advective and Kreiss-Oliger along each axis; for alpha,
@@ -443,40 +466,57 @@ __device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, i
 // flagged-face point the tail is the whole output, the Sommerfeld value
 // (bssn/src/rhs.cpp:833-977) plus sigma KO.
 // NFLD: 24 (BSSN) or 25 (CCZ4: Theta last, with its first derivatives staged).
-template <typename W, bool UPWIND, int NFLD>
+// The launch covers tiles tile0 .. tile0 + gridDim.x - 1, whose workspace
+// tiles start at wsoff.
+template <typename W, bool UPWIND, int NFLD, bool GEN>
 __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
-                                                     double *__restrict__ wsT, Params P, uint32_t tile0)
+                                                     double *__restrict__ wsT, Params P, uint32_t tile0,
+                                                     uint32_t wsoff)
 {
     constexpr int NT = 512;
-    constexpr int NHALO = TH * TH * TH;            // 2744 halo elements
-    // LDS rows of the halo tile are RS doubles apart: 48 dwords, so the rows
-    // a wave reads in one LDS pass (8 x-values each) fall on disjoint banks
-    constexpr int RS = 24, PS = RS * TH;
-    constexpr int NPF = (NHALO + NT - 1) / NT;     // prefetch registers per thread
-    __shared__ double U[PS * TH];        // x + RS (y + 14 z), halo coordinates
-    __shared__ double DX[T * TH * TH];   // D_x u: x in tile, y and z with halo
-    __shared__ double DY[T * T * TH];    // D_y u: x, y in tile, z with halo
+    // 8^3: LDS rows of the halo tile are 24 doubles apart (48 dwords), so the
+    // rows a wave reads in one LDS pass (8 x-values each) fall on disjoint banks
+    constexpr int NPF = ((GEN ? GEN_U_CAP : TH * TH * TH) + NT - 1) / NT;  // prefetch registers per thread
+    __shared__ double U[GEN ? GEN_U_CAP : 24 * TH * TH];  // halo tile: x + RS y + PS z
+    __shared__ double DX[GEN ? GEN_DX_CAP : T * TH * TH];  // D_x u: x in tile, y and z with halo
+    __shared__ double DY[GEN ? GEN_DY_CAP : T * T * TH];   // D_y u: x, y in tile, z with halo
     __shared__ double GD[3 * TILE_PTS];  // d_j (d_k beta^k) accumulated over the shift fields
 
     const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
     const uint32_t tile = tile0 + ct;
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
+    // tile extents, halo extents and LDS strides (constants for 8^3)
+    const int TX = GEN ? B.tdim[0] : T, TY = GEN ? B.tdim[1] : T, TZ = GEN ? B.tdim[2] : T;
+    const int HX = TX + 2 * GW, HY = TY + 2 * GW;
+    const int RS = GEN ? HX : 24, PS = GEN ? HX * HY : 24 * TH;
+    const int DXZ = TX * HY, DYZ = TX * TY;  // z strides of the D_x and D_y stages
+    const int NHALO = HX * HY * (TZ + 2 * GW), NPTS = TX * TY * TZ;
+    const Div dTX = make_div(TX), dTY = make_div(TY), dHX = make_div(HX), dHY = make_div(HY);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
     const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
     const AxisConst &a0 = B.ax[0], &a1 = B.ax[1], &a2 = B.ax[2];
     // face closures apply to the three points next to a flagged face: only
     // tiles that reach them take the per-point closure branches (uniform)
-    const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + T - 1 >= n0 - GW - 3;
-    const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + T - 1 >= n1 - GW - 3;
-    const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + T - 1 >= n2 - GW - 3;
+    const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + TX - 1 >= n0 - GW - 3;
+    const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + TY - 1 >= n1 - GW - 3;
+    const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + TZ - 1 >= n2 - GW - 3;
 
     const int t = threadIdx.x;
-    const int lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    int lx, ly, lz;
+    if (GEN) {
+        const int r = div_q(t, dTX);
+        lx = t - r * TX;
+        lz = div_q(r, dTY);
+        ly = r - lz * TY;
+    } else {
+        lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+    }
+    const bool has_pt = !GEN || t < NPTS;  // this thread has a point of the tile
     const int pi = tc.i0 + lx, pj = tc.j0 + ly, pk = tc.k0 + lz;
-    const bool valid = pi < n0 - GW && pj < n1 - GW && pk < n2 - GW;
+    const bool valid = has_pt && pi < n0 - GW && pj < n1 - GW && pk < n2 - GW;
     const bool face = valid && ((lo0 && pi == GW) || (hi0 && pi == n0 - GW - 1) || (lo1 && pj == GW) ||
                                 (hi1 && pj == n1 - GW - 1) || (lo2 && pk == GW) || (hi2 && pk == n2 - GW - 1));
     const uint64_t wp = (uint64_t)t;  // the point's position in the tile (x fastest)
@@ -495,8 +535,8 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
         for (int q = 0; q < 6; q++) g[q] = valid ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
         inverse_metric(g, igp);
     }
-    W *__restrict__ wsb = ws + ws_tile_base(ct, rs_nslot(NFLD));
-    double *__restrict__ tb = wsT + ws_tile_base(ct, NFLD);
+    W *__restrict__ wsb = ws + ws_tile_base(ct + wsoff, rs_nslot(NFLD));
+    double *__restrict__ tb = wsT + ws_tile_base(ct + wsoff, NFLD);
     constexpr uint64_t SS = WS_SS;
 
     // halo element -> offset in the block (or -1 outside the padded block)
@@ -504,15 +544,25 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
 #pragma unroll
     for (int q = 0; q < NPF; q++) {
         const int e = t + NT * q;
-        const int ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
+        int ex, ey, ez;
+        if (GEN) {
+            const int r = div_q(e, dHX);
+            ex = e - r * HX;
+            ez = div_q(r, dHY);
+            ey = r - ez * HY;
+        } else {
+            ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
+        }
         const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
         src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
     }
+    // halo element e = x + HX (y + HY z) -> its LDS place
+    auto lds_at = [](int e) { return GEN ? e : lds_halo(e); };
     {
         const double *__restrict__ s0 = in.p[0] + B.offset;
 #pragma unroll
         for (int q = 0; q < NPF; q++)
-            if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
+            if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
     }
     __syncthreads();
 
@@ -527,11 +577,11 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
             for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
         }
         const bool has_d2 = d2q(f) >= 0;  // alpha, chi, the shift, gt (uniform)
-        const double *c = &U[(lx + GW) + RS * ((ly + GW) + TH * (lz + GW))];
-        const int cx = lx + T * ((ly + GW) + TH * (lz + GW)), cy = lx + T * (ly + T * (lz + GW));
+        const double *c = &U[(lx + GW) + RS * (ly + GW) + PS * (lz + GW)];
+        const int cx = lx + TX * (ly + GW) + DXZ * (lz + GW), cy = lx + TX * ly + DYZ * (lz + GW);
         double h0 = 0.0, h1 = 0.0, h2 = 0.0;  // pure second derivatives xx, yy, zz
         if (!valid) {
-            if (has_d2) {  // a tile point outside the interior: the stages' own conditions
+            if (has_d2 && has_pt) {  // a tile point outside the interior: the stages' own conditions
                 DX[cx] = pi < n0 - GW ? d1(c, 1, pi, n0, clo0, chi0, a0) : 0.0;
                 DY[cy] = pi < n0 - GW && pj < n1 - GW ? d1(c, RS, pj, n1, clo1, chi1, a1) : 0.0;
             }
@@ -599,6 +649,34 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
             // (NA), and the tile's y rows of the z halo planes (NB); the
             // y-halo x z-halo corners are never read.  D_y u on the z halo
             // planes.
+            if (GEN) {
+                // one pass: each cell of the z halo planes (NB) gets both D_x
+                // and D_y; the y halo rows of the tile's z planes (NA) D_x
+                const int NA = TX * (2 * GW) * TZ, NB = TX * TY * (2 * GW);
+                for (int e = t; e < NA + NB; e += NT) {
+                    if (e < NB) {
+                        const int r = div_q(e, dTX), rz = div_q(r, dTY);
+                        const int xx = e - r * TX, yy = r - rz * TY, zz = rz < GW ? rz : rz + TZ;
+                        const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
+                        const double *cu = &U[(xx + GW) + RS * (yy + GW) + PS * zz];
+                        double vx = 0.0, vy = 0.0;
+                        if (gi < n0 - GW && gk < n2) {
+                            if (gj < n1) vx = d1(cu, 1, gi, n0, clo0, chi0, a0);
+                            if (gj < n1 - GW) vy = d1(cu, RS, gj, n1, clo1, chi1, a1);
+                        }
+                        DX[xx + TX * (yy + GW) + DXZ * zz] = vx;
+                        DY[xx + TX * yy + DYZ * zz] = vy;
+                    } else {
+                        const int r0 = e - NB, r = div_q(r0, dTX), zt = r / (2 * GW), ry = r - zt * (2 * GW);
+                        const int xx = r0 - r * TX, yy = ry < GW ? ry : ry + TY, zz = GW + zt;
+                        const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
+                        double v = 0.0;
+                        if (gi < n0 - GW && gj < n1 && gk < n2)
+                            v = d1(&U[(xx + GW) + RS * yy + PS * zz], 1, gi, n0, clo0, chi0, a0);
+                        DX[xx + TX * yy + DXZ * zz] = v;
+                    }
+                }
+            } else {
             constexpr int NA = T * (2 * GW) * T, NB = T * T * (2 * GW);
             for (int e = t; e < NA + NB; e += NT) {
                 int xx, yy, zz;
@@ -628,13 +706,14 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
                     v = d1(&U[(xx + GW) + RS * ((yy + GW) + TH * zz)], RS, gj, n1, clo1, chi1, a1);
                 DY[xx + T * (yy + T * zz)] = v;
             }
+            }
             __syncthreads();
             if (valid) {
-                const double *cxp = &DX[lx + T * ((ly + GW) + TH * (lz + GW))];
-                const double *cyp = &DY[lx + T * (ly + T * (lz + GW))];
-                const double m01 = d1(cxp, T, pj, n1, clo1, chi1, a1);       // D_y D_x
-                const double m02 = d1(cxp, T * TH, pk, n2, clo2, chi2, a2);  // D_z D_x
-                const double m12 = d1(cyp, T * T, pk, n2, clo2, chi2, a2);   // D_z D_y
+                const double *cxp = &DX[cx];
+                const double *cyp = &DY[cy];
+                const double m01 = d1(cxp, TX, pj, n1, clo1, chi1, a1);   // D_y D_x
+                const double m02 = d1(cxp, DXZ, pk, n2, clo2, chi2, a2);  // D_z D_x
+                const double m12 = d1(cyp, DYZ, pk, n2, clo2, chi2, a2);  // D_z D_y
                 if (f == 0 || f == 1) {
                     ws_put(&wsb[rs_d2(f, 1) * SS + wp], (W)m01);
                     ws_put(&wsb[rs_d2(f, 2) * SS + wp], (W)m02);
@@ -651,7 +730,7 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
                         const double r0 = f == 6 ? h0 : (f == 7 ? m01 : m02);
                         const double r1 = f == 6 ? m01 : (f == 7 ? h1 : m12);
                         const double r2 = f == 6 ? m02 : (f == 7 ? m12 : h2);
-                        double *gd = &GD[wp];
+                        double *gd = &GD[t];
                         if (f == 6) {
                             gd[0] = r0;
                             gd[TILE_PTS] = r1;
@@ -673,7 +752,7 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
         if (f + 1 < NFLD) {
 #pragma unroll
             for (int q = 0; q < NPF; q++)
-                if (t + NT * q < NHALO) U[lds_halo(t + NT * q)] = pf[q];
+                if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = pf[q];
         }
         __syncthreads();
     }
@@ -697,6 +776,29 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
 #undef SGR_PW_NFLD
 
 // ------------------------------------------------------------------ launch
+// The stencil pass of one launch: the 8^3-shaped tiles (the first a.nfast)
+// with the constant-indexed kernel, the others with the general one; events
+// ev[0] / ev[1] bracket the pair.
+template <typename W, bool UPWIND, int NFLD>
+static hipError_t launch_stencil(const LaunchArgs &a, W *ws, hipStream_t s)
+{
+    const uint32_t ngen = a.ntiles - a.nfast;
+    if (a.nfast) {
+        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, false>), dim3(a.nfast), dim3(512), 0, s, a.ev[0],
+                              ngen ? nullptr : a.ev[1], 0, a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0,
+                              0u);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (ngen) {
+        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, true>), dim3(ngen), dim3(512), 0, s,
+                              a.nfast ? nullptr : a.ev[0], a.ev[1], 0, a.in, a.blocks, a.tile_block, ws, a.wsT,
+                              a.params, a.tile0 + a.nfast, a.nfast);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
 template <typename R, typename W, bool UPWIND, bool RK>
 static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
 {
@@ -705,17 +807,13 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
     if (a.ccz4) {
-        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NF_CCZ4>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0,
-                              a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
-        hipError_t e = hipGetLastError();
+        hipError_t e = launch_stencil<W, UPWIND, NF_CCZ4>(a, ws, s);
         if (e != hipSuccess) return e;
         hipExtLaunchKernelGGL((k_ccz4_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                               a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
         return hipGetLastError();
     }
-    hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NF>), dim3(a.ntiles), dim3(512), 0, s, a.ev[0], a.ev[1], 0, a.in,
-                          a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_stencil<W, UPWIND, NF>(a, ws, s);
     if (e != hipSuccess) return e;
     hipExtLaunchKernelGGL((k_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
@@ -746,7 +844,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
     const uint32_t tile = tile0 + (wg >> 1);
     const int l = (int)((wg & 1) * 256 + threadIdx.x);
     const BlockDesc &B = blocks[tile_block[tile]];
-    const TileCoord tc = tile_coord(B, tile);
+    const TileCoord tc = tile_coord8(B, tile);
     const int i = tc.i0 + (l & 7), j = tc.j0 + ((l >> 3) & 7), k = tc.k0 + (l >> 6);
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;

@@ -19,7 +19,11 @@ __global__ __launch_bounds__(256, 2) void SGR_PW_NAME(FieldPtrs in, OutPtrs out,
     const int l = (int)((wg & 1) * 256 + threadIdx.x);
     const BlockDesc &B = blocks[tile_block[tile]];
     const TileCoord tc = tile_coord(B, tile);
-    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    // point l of the tile, x fastest (the stencil pass's order)
+    const int TX = B.tdim[0], TY = B.tdim[1];
+    if (l >= TX * TY * B.tdim[2]) return;
+    const int r = div_q(l, make_div(TX)), lz = div_q(r, make_div(TY));
+    const int lx = l - r * TX, ly = r - lz * TY;
     const int i = tc.i0 + lx, j = tc.j0 + ly, k = tc.k0 + lz;
     const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
     if (i >= n0 - GW || j >= n1 - GW || k >= n2 - GW) return;
