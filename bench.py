@@ -429,6 +429,42 @@ def interior_blocks_leg(bl, u, steps, hev):
     return res
 
 
+def dendro_sizes_section(hev, sizes=(9, 13, 17, 25, 33), steps=10):
+    """Block sizes of Dendro's own meshes (2^k p + 1 interior points): one
+    RHS in FP64 over a uniform tiling of about 12 M interior points per
+    size, with the tuned tile shape of the size (the library's table) and
+    with 8^3 tiles forced (SGR_TILE_SHAPE)."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    res = {}
+    for n in sizes:
+        nb = max(2, round((12e6 / n ** 3) ** (1.0 / 3.0)))
+        bl = data.uniform_tiling(nb=nb, n=n, stagger=0.5)
+        u = data.fill_blocks(bl, backend="torch", device="cuda")
+        out = torch.zeros_like(u)
+        pts = bl.interior_points()
+        r = {"blocks": len(bl), "interior_points": pts}
+        for tag, force in (("tuned", None), ("cube8", "8,8,8")):
+            if force:
+                os.environ["SGR_TILE_SHAPE"] = force
+            try:
+                plan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64", ko_sigma=0.1))
+            finally:
+                os.environ.pop("SGR_TILE_SHAPE", None)
+            w, kd, kp = timed_steps(plan, out, u, steps, 2, hev)
+            r[tag] = {"value": pts * steps / w / 1e6, "tiles": plan.info().tiles,
+                      "kernel_ms": {"k_stencil": kd, "k_pointwise": kp}}
+            del plan
+        r["gain"] = r["tuned"]["value"] / r["cube8"]["value"]
+        log(f"dendro size {n}^3: tuned tiles {r['tuned']['value']:.1f} Mpts/s, 8^3 tiles {r['cube8']['value']:.1f}")
+        res[str(n)] = r
+        del u, out
+        torch.cuda.empty_cache()
+    return {"workload": "uniform tilings of n^3-interior blocks (about 12 M interior points each), Brill-Lindquist, "
+                        "FP64, upwind, KO sigma 0.1", "sizes": res}
+
+
 def host_dropin_rate(bl, u, nsample=16, reps=3):
     """The C++ drop-in path on HOST arrays (sgr_bssnrhs_host, as bssnrhs(...)
     is called by a Dendro driver): one call per block over the first
@@ -569,6 +605,8 @@ def main():
     ap.add_argument("--no-octree", action="store_true",
                     help="skip the batched uneven-octree measurements (RHS, RK4 step, drift) and the large block")
     ap.add_argument("--no-ccz4", action="store_true", help="skip the CCZ4 192^3-block leg")
+    ap.add_argument("--no-dendro-sizes", action="store_true",
+                    help="skip the leg over Dendro's block sizes (9/13/17/25/33, tuned tiles against 8^3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (k_hbm_copy, reference point for HBM counters)")
     ap.add_argument("--counters", action="store_true",
@@ -578,7 +616,7 @@ def main():
                     help="precision of the RHS steps of a --counters run")
     args = ap.parse_args()
     if args.counters:
-        args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = args.no_ccz4 = True
+        args.no_mixed = args.no_drift = args.no_octree = args.no_cpu = args.no_ccz4 = args.no_dendro_sizes = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the RHS path is single-GPU here); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -718,6 +756,9 @@ def main():
     ccz4 = None
     if not args.no_ccz4:
         ccz4 = ccz4_section(hev)
+    dendro = None
+    if not args.no_dendro_sizes:
+        dendro = dendro_sizes_section(hev)
 
     if args.copy_calibration:
         import numpy as np
@@ -757,6 +798,7 @@ def main():
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": prec0},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
         "constraint_drift": drift, "large_block": large, "ccz4_large_block": ccz4, "interior_blocks": interior,
+        "dendro_block_sizes": dendro,
         "host_dropin": host,
     }
     print(json.dumps(line), flush=True)
