@@ -27,6 +27,8 @@ struct LaunchArgs {
     uint32_t ntiles;             // tiles of this launch (one workspace chunk)
     uint32_t tile0;              // first tile of this launch
     uint32_t nfast;              // RHS: the first nfast tiles of the launch have the 8^3 shape, the rest others
+    uint32_t *queue;             // RHS: 8 tile-queue counters (device; null: one stencil workgroup per tile)
+    uint32_t grid_cap;           // RHS: resident stencil workgroups (2 per CU), a multiple of 8
     void *ws;                    // derivative workspace (double or float), tile-major
     double *wsT;                 // RHS pass: tails of the pointwise outputs (one per field and point)
     double *wsK;                 // constraint pass: KO sums (24 per point)

@@ -466,13 +466,13 @@ __device__ __forceinline__ void axis_stencils(const double *c, int s, int pos, i
 // flagged-face point the tail is the whole output, the Sommerfeld value
 // (bssn/src/rhs.cpp:833-977) plus sigma KO.
 // NFLD: 24 (BSSN) or 25 (CCZ4: Theta last, with its first derivatives staged).
-// The launch covers tiles tile0 .. tile0 + gridDim.x - 1, whose workspace
-// tiles start at wsoff.
-template <typename W, bool UPWIND, int NFLD, bool GEN>
+// The launch covers tiles tile0 .. tile0 + ntl - 1, whose workspace tiles
+// start at wsoff.
+template <typename W, bool UPWIND, int NFLD, bool GEN, bool QUEUE>
 __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, W *__restrict__ ws,
                                                      double *__restrict__ wsT, Params P, uint32_t tile0,
-                                                     uint32_t wsoff)
+                                                     uint32_t wsoff, uint32_t ntl, uint32_t *__restrict__ queue)
 {
     constexpr int NT = 512;
     // 8^3: LDS rows of the halo tile are 24 doubles apart (48 dwords), so the
@@ -483,278 +483,315 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
     __shared__ double DY[GEN ? GEN_DY_CAP : T * T * TH];   // D_y u: x, y in tile, z with halo
     __shared__ double GD[3 * TILE_PTS];  // d_j (d_k beta^k) accumulated over the shift fields
 
-    const uint32_t ct = xcd_remap(blockIdx.x, gridDim.x);  // tile within this launch
-    const uint32_t tile = tile0 + ct;
-    const BlockDesc &B = blocks[tile_block[tile]];
-    const TileCoord tc = tile_coord(B, tile);
-    // tile extents, halo extents and LDS strides (constants for 8^3)
-    const int TX = GEN ? B.tdim[0] : T, TY = GEN ? B.tdim[1] : T, TZ = GEN ? B.tdim[2] : T;
-    const int HX = TX + 2 * GW, HY = TY + 2 * GW;
-    const int RS = GEN ? HX : 24, PS = GEN ? HX * HY : 24 * TH;
-    const int DXZ = TX * HY, DYZ = TX * TY;  // z strides of the D_x and D_y stages
-    const int NHALO = HX * HY * (TZ + 2 * GW), NPTS = TX * TY * TZ;
-    const Div dTX = make_div(TX), dTY = make_div(TY), dHX = make_div(HX), dHY = make_div(HY);
-    const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
-    const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
-    const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
-    const AxisConst &a0 = B.ax[0], &a1 = B.ax[1], &a2 = B.ax[2];
-    // face closures apply to the three points next to a flagged face: only
-    // tiles that reach them take the per-point closure branches (uniform)
-    const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + TX - 1 >= n0 - GW - 3;
-    const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + TY - 1 >= n1 - GW - 3;
-    const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + TZ - 1 >= n2 - GW - 3;
-
-    const int t = threadIdx.x;
-    int lx, ly, lz;
-    if (GEN) {
-        const int r = div_q(t, dTX);
-        lx = t - r * TX;
-        lz = div_q(r, dTY);
-        ly = r - lz * TY;
-    } else {
-        lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
-    }
-    const bool has_pt = !GEN || t < NPTS;  // this thread has a point of the tile
-    const int pi = tc.i0 + lx, pj = tc.j0 + ly, pk = tc.k0 + lz;
-    const bool valid = has_pt && pi < n0 - GW && pj < n1 - GW && pk < n2 - GW;
-    const bool face = valid && ((lo0 && pi == GW) || (hi0 && pi == n0 - GW - 1) || (lo1 && pj == GW) ||
-                                (hi1 && pj == n1 - GW - 1) || (lo2 && pk == GW) || (hi2 && pk == n2 - GW - 1));
-    const uint64_t wp = (uint64_t)t;  // the point's position in the tile (x fastest)
-    const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk);
-    const double sigma = P.ko_sigma;
-    // advection coefficient of each equation's own field (the tails)
-    const double c_alpha = (double)P.lambda[0];
-    const double c_beta = P.rochester ? (double)P.xi[1] : (double)P.lambda[1];
-    double bv[3];     // shift at the point
-    double igp[6];    // inverse conformal metric at the point (identity outside the interior)
-    {
-        double g[6];
-#pragma unroll
-        for (int d = 0; d < 3; d++) bv[d] = valid ? in.p[6 + d][gp] : 0.0;
-#pragma unroll
-        for (int q = 0; q < 6; q++) g[q] = valid ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
-        inverse_metric(g, igp);
-    }
-    W *__restrict__ wsb = ws + ws_tile_base(ct + wsoff, rs_nslot(NFLD));
-    double *__restrict__ tb = wsT + ws_tile_base(ct + wsoff, NFLD);
-    constexpr uint64_t SS = WS_SS;
-
-    // halo element -> offset in the block (or -1 outside the padded block)
-    int32_t src[NPF];
-#pragma unroll
-    for (int q = 0; q < NPF; q++) {
-        const int e = t + NT * q;
-        int ex, ey, ez;
-        if (GEN) {
-            const int r = div_q(e, dHX);
-            ex = e - r * HX;
-            ez = div_q(r, dHY);
-            ey = r - ez * HY;
-        } else {
-            ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
-        }
-        const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
-        src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
-    }
-    // halo element e = x + HX (y + HY z) -> its LDS place
-    auto lds_at = [](int e) { return GEN ? e : lds_halo(e); };
-    {
-        const double *__restrict__ s0 = in.p[0] + B.offset;
-#pragma unroll
-        for (int q = 0; q < NPF; q++)
-            if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
-    }
-    __syncthreads();
-
-#pragma unroll 1
-    for (int f = 0; f < NFLD; f++) {
-        // fetch the next field's halo tile into registers; it lands in LDS
-        // after this field's stencils, so the HBM latency overlaps them
-        double pf[NPF];
-        if (f + 1 < NFLD) {
-            const double *__restrict__ sn = in.p[f + 1] + B.offset;
-#pragma unroll
-            for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
-        }
-        const bool has_d2 = d2q(f) >= 0;  // alpha, chi, the shift, gt (uniform)
-        const double *c = &U[(lx + GW) + RS * (ly + GW) + PS * (lz + GW)];
-        const int cx = lx + TX * (ly + GW) + DXZ * (lz + GW), cy = lx + TX * ly + DYZ * (lz + GW);
-        double h0 = 0.0, h1 = 0.0, h2 = 0.0;  // pure second derivatives xx, yy, zz
-        if (!valid) {
-            if (has_d2 && has_pt) {  // a tile point outside the interior: the stages' own conditions
-                DX[cx] = pi < n0 - GW ? d1(c, 1, pi, n0, clo0, chi0, a0) : 0.0;
-                DY[cy] = pi < n0 - GW && pj < n1 - GW ? d1(c, RS, pj, n1, clo1, chi1, a1) : 0.0;
-            }
-        } else {
-            double g0, g1, g2, q0, q1, q2, k0, k1, k2;
-            axis_stencils<UPWIND>(c, 1, pi, n0, clo0, chi0, bv[0], a0, has_d2, g0, q0, k0, h0);
-            axis_stencils<UPWIND>(c, RS, pj, n1, clo1, chi1, bv[1], a1, has_d2, g1, q1, k1, h1);
-            axis_stencils<UPWIND>(c, PS, pk, n2, clo2, chi2, bv[2], a2, has_d2, g2, q2, k2, h2);
-            if (has_d2) {  // the tile points' own cells of the mixed-term stages
-                DX[cx] = g0;
-                DY[cy] = g1;
-            }
-            double advf, kof;
-            {
-                SGR_EXACT
-                advf = (bv[0] * q0 + bv[1] * q1) + bv[2] * q2;
-                kof = (k0 + k1) + k2;
-            }
-            if (rs_has_d1(f)) {
-                ws_put(&wsb[rs_d1(f, 0) * SS + wp], (W)g0);
-                ws_put(&wsb[rs_d1(f, 1) * SS + wp], (W)g1);
-                ws_put(&wsb[rs_d1(f, 2) * SS + wp], (W)g2);
-            }
-            if (f <= 1) {  // pure second derivatives of alpha and chi
-                ws_put(&wsb[rs_d2(f, 0) * SS + wp], (W)h0);
-                ws_put(&wsb[rs_d2(f, 3) * SS + wp], (W)h1);
-                ws_put(&wsb[rs_d2(f, 5) * SS + wp], (W)h2);
-            }
-            if (face) {
-                // the Sommerfeld value replaces the right-hand side, then + sigma KO
-                const double x = coord(B.pmin[0], pi, a0.h), y = coord(B.pmin[1], pj, a1.h);
-                const double z = coord(B.pmin[2], pk, a2.h);
-                ws_put(&tb[rs_tail(f) * SS + wp],
-                       ko_add(sommerfeld(x, y, z, 1.0 / radius(x, y, z), g0, g1, g2, c[0], bc_falloff(f), bc_asym(f)),
-                              sigma, kof));
-            } else if (f >= 3 && f <= 5) {
-                // Gt^i: its tail, and the Gt part of B^i's tail (completed at B^i)
-                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(advf, sigma, kof));
-                {
-                    SGR_EXACT
-                    ws_put(&tb[rs_tail(f + 6) * SS + wp], (1.0 - (double)P.lambda[3]) * advf);
-                }
-            } else if (f >= 9 && f <= 11) {
-                double v = 0.0;  // Rochester gauge: B is not evolved
-                if (!P.rochester) {
-                    SGR_EXACT
-                    v = ws_get(&tb[rs_tail(f) * SS + wp]) + (double)P.lambda[2] * advf;
-                }
-                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
-            } else {
-                const double cf = f == 0 ? c_alpha : (f >= 6 && f <= 8) ? c_beta : 1.0;
-                double v;
-                {
-                    SGR_EXACT
-                    v = cf * advf;
-                }
-                ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
-            }
-        }
-
-        if (has_d2) {  // uniform over the workgroup
-            // inner derivatives of the mixed terms, valid wherever the block
-            // has them.  D_x u only where D_y and D_z read it and the tile
-            // points did not write it: the y halo rows of the tile's z planes
-            // (NA), and the tile's y rows of the z halo planes (NB); the
-            // y-halo x z-halo corners are never read.  D_y u on the z halo
-            // planes.
-            if (GEN) {
-                // one pass: each cell of the z halo planes (NB) gets both D_x
-                // and D_y; the y halo rows of the tile's z planes (NA) D_x
-                const int NA = TX * (2 * GW) * TZ, NB = TX * TY * (2 * GW);
-                for (int e = t; e < NA + NB; e += NT) {
-                    if (e < NB) {
-                        const int r = div_q(e, dTX), rz = div_q(r, dTY);
-                        const int xx = e - r * TX, yy = r - rz * TY, zz = rz < GW ? rz : rz + TZ;
-                        const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
-                        const double *cu = &U[(xx + GW) + RS * (yy + GW) + PS * zz];
-                        double vx = 0.0, vy = 0.0;
-                        if (gi < n0 - GW && gk < n2) {
-                            if (gj < n1) vx = d1(cu, 1, gi, n0, clo0, chi0, a0);
-                            if (gj < n1 - GW) vy = d1(cu, RS, gj, n1, clo1, chi1, a1);
-                        }
-                        DX[xx + TX * (yy + GW) + DXZ * zz] = vx;
-                        DY[xx + TX * yy + DYZ * zz] = vy;
-                    } else {
-                        const int r0 = e - NB, r = div_q(r0, dTX), zt = r / (2 * GW), ry = r - zt * (2 * GW);
-                        const int xx = r0 - r * TX, yy = ry < GW ? ry : ry + TY, zz = GW + zt;
-                        const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
-                        double v = 0.0;
-                        if (gi < n0 - GW && gj < n1 && gk < n2)
-                            v = d1(&U[(xx + GW) + RS * yy + PS * zz], 1, gi, n0, clo0, chi0, a0);
-                        DX[xx + TX * yy + DXZ * zz] = v;
+    // !QUEUE: one tile per workgroup, in XCD-contiguous order (the loop runs
+    // once; its own instantiation keeps the single-tile kernel's register
+    // allocation).  QUEUE: a persistent grid fed by per-XCD tile queues: the launch's
+    // ntl tiles are split into 8 contiguous ranges, one per XCD (workgroups
+    // are dispatched to XCDs round-robin by id, so workgroup b takes from
+    // queue b mod 8 first, keeping neighbouring tiles, which share halo data,
+    // in one L2), and a workgroup whose range is exhausted takes from the
+    // others.  queue[0..7] are zero at launch; every workgroup leaves when all
+    // eight ranges are exhausted.
+    __shared__ uint32_t s_ct;
+    for (;;) {
+        uint32_t ct;  // tile within this launch
+        if constexpr (QUEUE) {
+            if (threadIdx.x == 0) {
+                uint32_t v = 0xFFFFFFFFu;
+                for (uint32_t i = 0; i < 8; i++) {
+                    const uint32_t xq = (blockIdx.x + i) & 7;
+                    const uint32_t lo = (uint32_t)(((uint64_t)ntl * xq) >> 3);
+                    const uint32_t hi = (uint32_t)(((uint64_t)ntl * (xq + 1)) >> 3);
+                    if (lo >= hi) continue;
+                    const uint32_t k = atomicAdd(&queue[xq], 1u);
+                    if (lo + k < hi) {
+                        v = lo + k;
+                        break;
                     }
                 }
-            } else {
-            constexpr int NA = T * (2 * GW) * T, NB = T * T * (2 * GW);
-            for (int e = t; e < NA + NB; e += NT) {
-                int xx, yy, zz;
-                if (e < NA) {
-                    const int ry = (e / T) % (2 * GW);
-                    xx = e % T;
-                    yy = ry < GW ? ry : ry + T;
-                    zz = GW + e / (T * 2 * GW);
-                } else {
-                    const int r = e - NA, rz = r / (T * T);
-                    xx = r % T;
-                    yy = GW + (r / T) % T;
-                    zz = rz < GW ? rz : rz + T;
-                }
-                const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
-                double v = 0.0;
-                if (gi < n0 - GW && gj < n1 && gk < n2)
-                    v = d1(&U[(xx + GW) + RS * (yy + TH * zz)], 1, gi, n0, clo0, chi0, a0);
-                DX[xx + T * (yy + TH * zz)] = v;
-            }
-            for (int e = t; e < T * T * (2 * GW); e += NT) {
-                const int rz = e / (T * T);
-                const int xx = e % T, yy = (e / T) % T, zz = rz < GW ? rz : rz + T;
-                const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
-                double v = 0.0;
-                if (gi < n0 - GW && gj < n1 - GW && gk < n2)
-                    v = d1(&U[(xx + GW) + RS * ((yy + GW) + TH * zz)], RS, gj, n1, clo1, chi1, a1);
-                DY[xx + T * (yy + T * zz)] = v;
-            }
+                s_ct = v;
             }
             __syncthreads();
-            if (valid) {
-                const double *cxp = &DX[cx];
-                const double *cyp = &DY[cy];
-                const double m01 = d1(cxp, TX, pj, n1, clo1, chi1, a1);   // D_y D_x
-                const double m02 = d1(cxp, DXZ, pk, n2, clo2, chi2, a2);  // D_z D_x
-                const double m12 = d1(cyp, DYZ, pk, n2, clo2, chi2, a2);  // D_z D_y
-                if (f == 0 || f == 1) {
-                    ws_put(&wsb[rs_d2(f, 1) * SS + wp], (W)m01);
-                    ws_put(&wsb[rs_d2(f, 2) * SS + wp], (W)m02);
-                    ws_put(&wsb[rs_d2(f, 4) * SS + wp], (W)m12);
-                } else {
-                    // the shift and gt_ij enter only contracted: igt^{lm} d_l d_m f
-                    // is stored, and for the shift d_j d_k beta^k is accumulated
+            ct = s_ct;
+            __syncthreads();  // every thread has read s_ct before it is written again
+            if (ct == 0xFFFFFFFFu) break;
+        } else {
+            ct = xcd_remap(blockIdx.x, gridDim.x);
+        }
+        const uint32_t tile = tile0 + ct;
+        const BlockDesc &B = blocks[tile_block[tile]];
+        const TileCoord tc = tile_coord(B, tile);
+        // tile extents, halo extents and LDS strides (constants for 8^3)
+        const int TX = GEN ? B.tdim[0] : T, TY = GEN ? B.tdim[1] : T, TZ = GEN ? B.tdim[2] : T;
+        const int HX = TX + 2 * GW, HY = TY + 2 * GW;
+        const int RS = GEN ? HX : 24, PS = GEN ? HX * HY : 24 * TH;
+        const int DXZ = TX * HY, DYZ = TX * TY;  // z strides of the D_x and D_y stages
+        const int NHALO = HX * HY * (TZ + 2 * GW), NPTS = TX * TY * TZ;
+        const Div dTX = make_div(TX), dTY = make_div(TY), dHX = make_div(HX), dHY = make_div(HY);
+        const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+        const bool lo0 = B.bflag & 1u, hi0 = B.bflag & 2u, lo1 = B.bflag & 4u, hi1 = B.bflag & 8u;
+        const bool lo2 = B.bflag & 16u, hi2 = B.bflag & 32u;
+        const AxisConst &a0 = B.ax[0], &a1 = B.ax[1], &a2 = B.ax[2];
+        // face closures apply to the three points next to a flagged face: only
+        // tiles that reach them take the per-point closure branches (uniform)
+        const bool clo0 = lo0 && tc.i0 < GW + 3, chi0 = hi0 && tc.i0 + TX - 1 >= n0 - GW - 3;
+        const bool clo1 = lo1 && tc.j0 < GW + 3, chi1 = hi1 && tc.j0 + TY - 1 >= n1 - GW - 3;
+        const bool clo2 = lo2 && tc.k0 < GW + 3, chi2 = hi2 && tc.k0 + TZ - 1 >= n2 - GW - 3;
+
+        const int t = threadIdx.x;
+        int lx, ly, lz;
+        if (GEN) {
+            const int r = div_q(t, dTX);
+            lx = t - r * TX;
+            lz = div_q(r, dTY);
+            ly = r - lz * TY;
+        } else {
+            lx = t & 7, ly = (t >> 3) & 7, lz = t >> 6;
+        }
+        const bool has_pt = !GEN || t < NPTS;  // this thread has a point of the tile
+        const int pi = tc.i0 + lx, pj = tc.j0 + ly, pk = tc.k0 + lz;
+        const bool valid = has_pt && pi < n0 - GW && pj < n1 - GW && pk < n2 - GW;
+        const bool face = valid && ((lo0 && pi == GW) || (hi0 && pi == n0 - GW - 1) || (lo1 && pj == GW) ||
+                                    (hi1 && pj == n1 - GW - 1) || (lo2 && pk == GW) || (hi2 && pk == n2 - GW - 1));
+        const uint64_t wp = (uint64_t)t;  // the point's position in the tile (x fastest)
+        const uint64_t gp = B.offset + pi + (uint64_t)n0 * (pj + (uint64_t)n1 * pk);
+        const double sigma = P.ko_sigma;
+        // advection coefficient of each equation's own field (the tails)
+        const double c_alpha = (double)P.lambda[0];
+        const double c_beta = P.rochester ? (double)P.xi[1] : (double)P.lambda[1];
+        double bv[3];     // shift at the point
+        double igp[6];    // inverse conformal metric at the point (identity outside the interior)
+        {
+            double g[6];
+    #pragma unroll
+            for (int d = 0; d < 3; d++) bv[d] = valid ? in.p[6 + d][gp] : 0.0;
+    #pragma unroll
+            for (int q = 0; q < 6; q++) g[q] = valid ? in.p[12 + q][gp] : (q == 0 || q == 3 || q == 5 ? 1.0 : 0.0);
+            inverse_metric(g, igp);
+        }
+        W *__restrict__ wsb = ws + ws_tile_base(ct + wsoff, rs_nslot(NFLD));
+        double *__restrict__ tb = wsT + ws_tile_base(ct + wsoff, NFLD);
+        constexpr uint64_t SS = WS_SS;
+
+        // halo element -> offset in the block (or -1 outside the padded block)
+        int32_t src[NPF];
+    #pragma unroll
+        for (int q = 0; q < NPF; q++) {
+            const int e = t + NT * q;
+            int ex, ey, ez;
+            if (GEN) {
+                const int r = div_q(e, dHX);
+                ex = e - r * HX;
+                ez = div_q(r, dHY);
+                ey = r - ez * HY;
+            } else {
+                ex = e % TH, ey = (e / TH) % TH, ez = e / (TH * TH);
+            }
+            const int gi = tc.i0 - GW + ex, gj = tc.j0 - GW + ey, gk = tc.k0 - GW + ez;
+            src[q] = (e < NHALO && gi < n0 && gj < n1 && gk < n2) ? gi + n0 * (gj + n1 * gk) : -1;
+        }
+        // halo element e = x + HX (y + HY z) -> its LDS place
+        auto lds_at = [](int e) { return GEN ? e : lds_halo(e); };
+        {
+            const double *__restrict__ s0 = in.p[0] + B.offset;
+    #pragma unroll
+            for (int q = 0; q < NPF; q++)
+                if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = src[q] >= 0 ? s0[src[q]] : 0.0;
+        }
+        __syncthreads();
+
+    #pragma unroll 1
+        for (int f = 0; f < NFLD; f++) {
+            // fetch the next field's halo tile into registers; it lands in LDS
+            // after this field's stencils, so the HBM latency overlaps them
+            double pf[NPF];
+            if (f + 1 < NFLD) {
+                const double *__restrict__ sn = in.p[f + 1] + B.offset;
+    #pragma unroll
+                for (int q = 0; q < NPF; q++) pf[q] = src[q] >= 0 ? sn[src[q]] : 0.0;
+            }
+            const bool has_d2 = d2q(f) >= 0;  // alpha, chi, the shift, gt (uniform)
+            const double *c = &U[(lx + GW) + RS * (ly + GW) + PS * (lz + GW)];
+            const int cx = lx + TX * (ly + GW) + DXZ * (lz + GW), cy = lx + TX * ly + DYZ * (lz + GW);
+            double h0 = 0.0, h1 = 0.0, h2 = 0.0;  // pure second derivatives xx, yy, zz
+            if (!valid) {
+                if (has_d2 && has_pt) {  // a tile point outside the interior: the stages' own conditions
+                    DX[cx] = pi < n0 - GW ? d1(c, 1, pi, n0, clo0, chi0, a0) : 0.0;
+                    DY[cy] = pi < n0 - GW && pj < n1 - GW ? d1(c, RS, pj, n1, clo1, chi1, a1) : 0.0;
+                }
+            } else {
+                double g0, g1, g2, q0, q1, q2, k0, k1, k2;
+                axis_stencils<UPWIND>(c, 1, pi, n0, clo0, chi0, bv[0], a0, has_d2, g0, q0, k0, h0);
+                axis_stencils<UPWIND>(c, RS, pj, n1, clo1, chi1, bv[1], a1, has_d2, g1, q1, k1, h1);
+                axis_stencils<UPWIND>(c, PS, pk, n2, clo2, chi2, bv[2], a2, has_d2, g2, q2, k2, h2);
+                if (has_d2) {  // the tile points' own cells of the mixed-term stages
+                    DX[cx] = g0;
+                    DY[cy] = g1;
+                }
+                double advf, kof;
+                {
                     SGR_EXACT
-                    const double *ig = igp;
-                    const double L = ((((ig[0] * h0 + 2.0 * (ig[1] * m01)) + 2.0 * (ig[2] * m02)) + ig[3] * h1) +
-                                      2.0 * (ig[4] * m12)) + ig[5] * h2;
-                    ws_put(&wsb[rs_lap(f) * SS + wp], (W)L);
-                    if (f <= 8) {  // shift beta^k, k = f - 6: d_j d_k beta^k for j = 0, 1, 2
-                        const double r0 = f == 6 ? h0 : (f == 7 ? m01 : m02);
-                        const double r1 = f == 6 ? m01 : (f == 7 ? h1 : m12);
-                        const double r2 = f == 6 ? m02 : (f == 7 ? m12 : h2);
-                        double *gd = &GD[t];
-                        if (f == 6) {
-                            gd[0] = r0;
-                            gd[TILE_PTS] = r1;
-                            gd[2 * TILE_PTS] = r2;
-                        } else if (f == 7) {
-                            gd[0] = gd[0] + r0;
-                            gd[TILE_PTS] = gd[TILE_PTS] + r1;
-                            gd[2 * TILE_PTS] = gd[2 * TILE_PTS] + r2;
+                    advf = (bv[0] * q0 + bv[1] * q1) + bv[2] * q2;
+                    kof = (k0 + k1) + k2;
+                }
+                if (rs_has_d1(f)) {
+                    ws_put(&wsb[rs_d1(f, 0) * SS + wp], (W)g0);
+                    ws_put(&wsb[rs_d1(f, 1) * SS + wp], (W)g1);
+                    ws_put(&wsb[rs_d1(f, 2) * SS + wp], (W)g2);
+                }
+                if (f <= 1) {  // pure second derivatives of alpha and chi
+                    ws_put(&wsb[rs_d2(f, 0) * SS + wp], (W)h0);
+                    ws_put(&wsb[rs_d2(f, 3) * SS + wp], (W)h1);
+                    ws_put(&wsb[rs_d2(f, 5) * SS + wp], (W)h2);
+                }
+                if (face) {
+                    // the Sommerfeld value replaces the right-hand side, then + sigma KO
+                    const double x = coord(B.pmin[0], pi, a0.h), y = coord(B.pmin[1], pj, a1.h);
+                    const double z = coord(B.pmin[2], pk, a2.h);
+                    ws_put(&tb[rs_tail(f) * SS + wp],
+                           ko_add(sommerfeld(x, y, z, 1.0 / radius(x, y, z), g0, g1, g2, c[0], bc_falloff(f), bc_asym(f)),
+                                  sigma, kof));
+                } else if (f >= 3 && f <= 5) {
+                    // Gt^i: its tail, and the Gt part of B^i's tail (completed at B^i)
+                    ws_put(&tb[rs_tail(f) * SS + wp], ko_add(advf, sigma, kof));
+                    {
+                        SGR_EXACT
+                        ws_put(&tb[rs_tail(f + 6) * SS + wp], (1.0 - (double)P.lambda[3]) * advf);
+                    }
+                } else if (f >= 9 && f <= 11) {
+                    double v = 0.0;  // Rochester gauge: B is not evolved
+                    if (!P.rochester) {
+                        SGR_EXACT
+                        v = ws_get(&tb[rs_tail(f) * SS + wp]) + (double)P.lambda[2] * advf;
+                    }
+                    ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
+                } else {
+                    const double cf = f == 0 ? c_alpha : (f >= 6 && f <= 8) ? c_beta : 1.0;
+                    double v;
+                    {
+                        SGR_EXACT
+                        v = cf * advf;
+                    }
+                    ws_put(&tb[rs_tail(f) * SS + wp], ko_add(v, sigma, kof));
+                }
+            }
+
+            if (has_d2) {  // uniform over the workgroup
+                // inner derivatives of the mixed terms, valid wherever the block
+                // has them.  D_x u only where D_y and D_z read it and the tile
+                // points did not write it: the y halo rows of the tile's z planes
+                // (NA), and the tile's y rows of the z halo planes (NB); the
+                // y-halo x z-halo corners are never read.  D_y u on the z halo
+                // planes.
+                if (GEN) {
+                    // one pass: each cell of the z halo planes (NB) gets both D_x
+                    // and D_y; the y halo rows of the tile's z planes (NA) D_x
+                    const int NA = TX * (2 * GW) * TZ, NB = TX * TY * (2 * GW);
+                    for (int e = t; e < NA + NB; e += NT) {
+                        if (e < NB) {
+                            const int r = div_q(e, dTX), rz = div_q(r, dTY);
+                            const int xx = e - r * TX, yy = r - rz * TY, zz = rz < GW ? rz : rz + TZ;
+                            const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
+                            const double *cu = &U[(xx + GW) + RS * (yy + GW) + PS * zz];
+                            double vx = 0.0, vy = 0.0;
+                            if (gi < n0 - GW && gk < n2) {
+                                if (gj < n1) vx = d1(cu, 1, gi, n0, clo0, chi0, a0);
+                                if (gj < n1 - GW) vy = d1(cu, RS, gj, n1, clo1, chi1, a1);
+                            }
+                            DX[xx + TX * (yy + GW) + DXZ * zz] = vx;
+                            DY[xx + TX * yy + DYZ * zz] = vy;
                         } else {
-                            ws_put(&wsb[rs_gdb(0) * SS + wp], (W)(gd[0] + r0));
-                            ws_put(&wsb[rs_gdb(1) * SS + wp], (W)(gd[TILE_PTS] + r1));
-                            ws_put(&wsb[rs_gdb(2) * SS + wp], (W)(gd[2 * TILE_PTS] + r2));
+                            const int r0 = e - NB, r = div_q(r0, dTX), zt = r / (2 * GW), ry = r - zt * (2 * GW);
+                            const int xx = r0 - r * TX, yy = ry < GW ? ry : ry + TY, zz = GW + zt;
+                            const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
+                            double v = 0.0;
+                            if (gi < n0 - GW && gj < n1 && gk < n2)
+                                v = d1(&U[(xx + GW) + RS * yy + PS * zz], 1, gi, n0, clo0, chi0, a0);
+                            DX[xx + TX * yy + DXZ * zz] = v;
+                        }
+                    }
+                } else {
+                constexpr int NA = T * (2 * GW) * T, NB = T * T * (2 * GW);
+                for (int e = t; e < NA + NB; e += NT) {
+                    int xx, yy, zz;
+                    if (e < NA) {
+                        const int ry = (e / T) % (2 * GW);
+                        xx = e % T;
+                        yy = ry < GW ? ry : ry + T;
+                        zz = GW + e / (T * 2 * GW);
+                    } else {
+                        const int r = e - NA, rz = r / (T * T);
+                        xx = r % T;
+                        yy = GW + (r / T) % T;
+                        zz = rz < GW ? rz : rz + T;
+                    }
+                    const int gi = tc.i0 + xx, gj = tc.j0 - GW + yy, gk = tc.k0 - GW + zz;
+                    double v = 0.0;
+                    if (gi < n0 - GW && gj < n1 && gk < n2)
+                        v = d1(&U[(xx + GW) + RS * (yy + TH * zz)], 1, gi, n0, clo0, chi0, a0);
+                    DX[xx + T * (yy + TH * zz)] = v;
+                }
+                for (int e = t; e < T * T * (2 * GW); e += NT) {
+                    const int rz = e / (T * T);
+                    const int xx = e % T, yy = (e / T) % T, zz = rz < GW ? rz : rz + T;
+                    const int gi = tc.i0 + xx, gj = tc.j0 + yy, gk = tc.k0 - GW + zz;
+                    double v = 0.0;
+                    if (gi < n0 - GW && gj < n1 - GW && gk < n2)
+                        v = d1(&U[(xx + GW) + RS * ((yy + GW) + TH * zz)], RS, gj, n1, clo1, chi1, a1);
+                    DY[xx + T * (yy + T * zz)] = v;
+                }
+                }
+                __syncthreads();
+                if (valid) {
+                    const double *cxp = &DX[cx];
+                    const double *cyp = &DY[cy];
+                    const double m01 = d1(cxp, TX, pj, n1, clo1, chi1, a1);   // D_y D_x
+                    const double m02 = d1(cxp, DXZ, pk, n2, clo2, chi2, a2);  // D_z D_x
+                    const double m12 = d1(cyp, DYZ, pk, n2, clo2, chi2, a2);  // D_z D_y
+                    if (f == 0 || f == 1) {
+                        ws_put(&wsb[rs_d2(f, 1) * SS + wp], (W)m01);
+                        ws_put(&wsb[rs_d2(f, 2) * SS + wp], (W)m02);
+                        ws_put(&wsb[rs_d2(f, 4) * SS + wp], (W)m12);
+                    } else {
+                        // the shift and gt_ij enter only contracted: igt^{lm} d_l d_m f
+                        // is stored, and for the shift d_j d_k beta^k is accumulated
+                        SGR_EXACT
+                        const double *ig = igp;
+                        const double L = ((((ig[0] * h0 + 2.0 * (ig[1] * m01)) + 2.0 * (ig[2] * m02)) + ig[3] * h1) +
+                                          2.0 * (ig[4] * m12)) + ig[5] * h2;
+                        ws_put(&wsb[rs_lap(f) * SS + wp], (W)L);
+                        if (f <= 8) {  // shift beta^k, k = f - 6: d_j d_k beta^k for j = 0, 1, 2
+                            const double r0 = f == 6 ? h0 : (f == 7 ? m01 : m02);
+                            const double r1 = f == 6 ? m01 : (f == 7 ? h1 : m12);
+                            const double r2 = f == 6 ? m02 : (f == 7 ? m12 : h2);
+                            double *gd = &GD[t];
+                            if (f == 6) {
+                                gd[0] = r0;
+                                gd[TILE_PTS] = r1;
+                                gd[2 * TILE_PTS] = r2;
+                            } else if (f == 7) {
+                                gd[0] = gd[0] + r0;
+                                gd[TILE_PTS] = gd[TILE_PTS] + r1;
+                                gd[2 * TILE_PTS] = gd[2 * TILE_PTS] + r2;
+                            } else {
+                                ws_put(&wsb[rs_gdb(0) * SS + wp], (W)(gd[0] + r0));
+                                ws_put(&wsb[rs_gdb(1) * SS + wp], (W)(gd[TILE_PTS] + r1));
+                                ws_put(&wsb[rs_gdb(2) * SS + wp], (W)(gd[2 * TILE_PTS] + r2));
+                            }
                         }
                     }
                 }
             }
+            __syncthreads();  // every read of U (and DX, DY) for field f is done
+            if (f + 1 < NFLD) {
+    #pragma unroll
+                for (int q = 0; q < NPF; q++)
+                    if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = pf[q];
+            }
+            __syncthreads();
         }
-        __syncthreads();  // every read of U (and DX, DY) for field f is done
-        if (f + 1 < NFLD) {
-#pragma unroll
-            for (int q = 0; q < NPF; q++)
-                if (t + NT * q < NHALO) U[lds_at(t + NT * q)] = pf[q];
-        }
-        __syncthreads();
+        if constexpr (!QUEUE) break;
+        __syncthreads();  // the tile's last reads of U, DX, DY and GD are done
     }
 }
 
@@ -776,26 +813,55 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
 #undef SGR_PW_NFLD
 
 // ------------------------------------------------------------------ launch
+__global__ void k_queue_reset(uint32_t *queue)
+{
+    if (threadIdx.x < 8) queue[threadIdx.x] = 0u;
+}
+
+// Stencil grid for n tiles: one workgroup per tile, or with the tile
+// queues the resident workgroups (a.grid_cap, a multiple of 8) and at least
+// 8 so that every XCD queue is served.
+static uint32_t stencil_grid(const LaunchArgs &a, uint32_t n)
+{
+    if (!a.queue) return n;
+    const uint32_t g = ((n + 7) / 8) * 8;
+    return g < a.grid_cap ? g : a.grid_cap;
+}
+
+// One stencil launch over n tiles from tile0 (workspace tiles from wsoff);
+// with the tile queues, after a kernel that zeroes them (a kernel, not a
+// memset, so that a captured graph replays it).
+template <typename W, bool UPWIND, int NFLD, bool GEN>
+static hipError_t launch_stencil1(const LaunchArgs &a, W *ws, uint32_t tile0, uint32_t wsoff, uint32_t n,
+                                  hipEvent_t e0, hipEvent_t e1, hipStream_t s)
+{
+    if (a.queue) {
+        hipExtLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, e0, nullptr, 0, a.queue);
+        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, GEN, true>), dim3(stencil_grid(a, n)), dim3(512), 0, s,
+                              nullptr, e1, 0, a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, tile0, wsoff, n,
+                              a.queue);
+    } else {
+        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, GEN, false>), dim3(n), dim3(512), 0, s, e0, e1, 0, a.in,
+                              a.blocks, a.tile_block, ws, a.wsT, a.params, tile0, wsoff, n, nullptr);
+    }
+    return hipGetLastError();
+}
+
 // The stencil pass of one launch: the 8^3-shaped tiles (the first a.nfast)
 // with the constant-indexed kernel, the others with the general one; events
-// ev[0] / ev[1] bracket the pair.
+// ev[0] / ev[1] bracket the pass.
 template <typename W, bool UPWIND, int NFLD>
 static hipError_t launch_stencil(const LaunchArgs &a, W *ws, hipStream_t s)
 {
     const uint32_t ngen = a.ntiles - a.nfast;
     if (a.nfast) {
-        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, false>), dim3(a.nfast), dim3(512), 0, s, a.ev[0],
-                              ngen ? nullptr : a.ev[1], 0, a.in, a.blocks, a.tile_block, ws, a.wsT, a.params, a.tile0,
-                              0u);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        const hipError_t e = launch_stencil1<W, UPWIND, NFLD, false>(a, ws, a.tile0, 0u, a.nfast, a.ev[0],
+                                                                     ngen ? nullptr : a.ev[1], s);
+        if (e != hipSuccess || !ngen) return e;
     }
-    if (ngen) {
-        hipExtLaunchKernelGGL((k_stencil<W, UPWIND, NFLD, true>), dim3(ngen), dim3(512), 0, s,
-                              a.nfast ? nullptr : a.ev[0], a.ev[1], 0, a.in, a.blocks, a.tile_block, ws, a.wsT,
-                              a.params, a.tile0 + a.nfast, a.nfast);
-        return hipGetLastError();
-    }
+    if (ngen)
+        return launch_stencil1<W, UPWIND, NFLD, true>(a, ws, a.tile0 + a.nfast, a.nfast, ngen,
+                                                      a.nfast ? nullptr : a.ev[0], a.ev[1], s);
     return hipSuccess;
 }
 

@@ -18,14 +18,14 @@ def test_allocation_failure_at_every_step_is_reported_and_cleaned_up(monkeypatch
     sgr.Plan(bl, sgr.params()).rhs(ref, u)
     torch.cuda.synchronize()
     free0 = torch.cuda.mem_get_info()[0]
-    # blocks, RHS tile map, constraint tile map, workspace, flag, ghost patches, ghost boxes
-    for k in range(1, 8):
+    # blocks, RHS tile map, constraint tile map, workspace, flag, tile queue, ghost patches, ghost boxes
+    for k in range(1, 9):
         monkeypatch.setenv("SGR_FAULT_ALLOC", str(k))
         print(f"fault at allocation {k}", flush=True)
         with pytest.raises(sgr.SgrError) as e:
             sgr.Plan(bl, sgr.params())
         assert e.value.code == -4 and "out of device memory" in str(e.value), str(e.value)
-    monkeypatch.setenv("SGR_FAULT_ALLOC", "8")  # past the last allocation: no fault
+    monkeypatch.setenv("SGR_FAULT_ALLOC", "9")  # past the last allocation: no fault
     plan = sgr.Plan(bl, sgr.params())
     monkeypatch.delenv("SGR_FAULT_ALLOC")
     out = torch.zeros_like(u)

@@ -155,3 +155,39 @@ def test_bad_tile_shapes_are_refused(monkeypatch):
         with pytest.raises(sgr.SgrError, match="SGR_TILE_SHAPE"):
             sgr.Plan(bl, sgr.params())
     monkeypatch.delenv("SGR_TILE_SHAPE", raising=False)
+
+
+@pytest.mark.parametrize("shape", [None, "13,13,3"])
+def test_stencil_queue_equals_one_workgroup_per_tile(shape, monkeypatch):
+    """The stencil pass as a persistent grid fed by the per-XCD tile queues
+    and as one workgroup per tile give the same RHS bit for bit, also on RK4
+    stages and over several launches (workspace chunks)."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    bl = data.uniform_tiling(nb=3, n=13 if shape else 16, lo=-1.0, hi=1.0, stagger=0.5, faces=True)
+    u = data.fill_blocks(bl, backend="torch", device="cuda", kind="perturbed", seed=8, amp=1e-2, lo=-1.0, L=2.0)
+    if shape:
+        monkeypatch.setenv("SGR_TILE_SHAPE", shape)
+    outs = []
+    for q, gib in (("0", None), ("1", None), ("1", "0.02")):
+        monkeypatch.setenv("SGR_STENCIL_QUEUE", q)
+        if gib:
+            monkeypatch.setenv("SGR_WS_GIB", gib)
+        print(f"queue={q} ws_gib={gib} shape={shape}", flush=True)
+        plan = sgr.Plan(bl, sgr.params(adv="upwind", ko_sigma=0.1))
+        out = torch.full_like(u, 5.0)
+        plan.rhs(out, u)
+        acc = torch.full_like(u, 3.0)
+        nxt = torch.zeros_like(u)
+        plan.rk4_stage(1, 0.01, u, u, acc, nxt)
+        torch.cuda.synchronize()
+        outs.append((out, acc, nxt, plan.info().chunks))
+        del plan
+        monkeypatch.delenv("SGR_WS_GIB", raising=False)
+    monkeypatch.delenv("SGR_STENCIL_QUEUE", raising=False)
+    monkeypatch.delenv("SGR_TILE_SHAPE", raising=False)
+    assert outs[2][3] > 1
+    for o in outs[1:]:
+        for k in range(3):
+            assert torch.equal(o[k], outs[0][k]), k
