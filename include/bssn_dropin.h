@@ -25,7 +25,8 @@ void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &
 
 /* The staged form, bssn/include/rhs.h:39-42 (bssn/src/rhs.cpp:392-825): the
  * same right-hand side computed one variable at a time; here the same
- * kernels serve it. */
+ * kernels serve it.  Like the reference it reads bssn::DISSIPATION_TYPE
+ * (0 Kreiss-Oliger, 1 TVB3, 2 TVB5; bssnrhs always uses Kreiss-Oliger). */
 void bssnrhs_sep(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
                  const double *ptmax, const unsigned int *sz, const unsigned int &bflag);
 

@@ -40,6 +40,7 @@ enum { SGR_PREC_FP64 = 0, SGR_PREC_MIXED = 1 };
 enum { SGR_GAUGE_STANDARD = 0, SGR_GAUGE_ROCHESTER = 1 };
 enum { SGR_ETA_CONST = 0, SGR_ETA_FUNC = 1 };
 enum { SGR_FORM_BSSN = 0, SGR_FORM_CCZ4 = 1 };
+enum { SGR_DISS_KO = 0, SGR_DISS_TVB3 = 1, SGR_DISS_TVB5 = 2 };
 
 /* Fields of a CCZ4 plan: the 24 BSSN fields (the Gt slots hold the evolved
  * Gamma-hat^i) and Theta as field 24.  Every field-array argument of a CCZ4
@@ -78,10 +79,20 @@ typedef struct {
     int32_t formulation;
     double kappa[3];          /* CCZ4 damping kappa1, kappa2, kappa3 (real; the
                                  reference declares them unsigned, ccz4_eqns.py:42-45) */
+    /* DISSIPATION_TYPE (bssn/src/parameters.cpp:109, rhs.cpp:729-757), scaled by
+       ko_sigma: SGR_DISS_KO (Kreiss-Oliger, ko_deriv42_*), SGR_DISS_TVB3 or
+       SGR_DISS_TVB5 (disstvb3_* / disstvb5_*, derivs.cpp:1348-2054, weighted by
+       the characteristic speeds of max_spacetime_speeds, rhs.cpp:984-1028).
+       TVB: BSSN only, not in sgr_rhs_block_noalloc.  Where the reference's TVB
+       result is undefined (speeds read at ghost nodes it never wrote, TVB3
+       values left unwritten next to an unflagged face) the speeds are
+       evaluated at ghost nodes too and TVB3 takes its interior form there. */
+    int32_t dissipation;
 } sgr_params;
 
 /* Defaults of bssn/src/parameters.cpp:50-51,99-109,123 with adv_mode/precision
- * given: standard gauge, constant eta, BSSN; kappa = (0.02, 0, 0.5). */
+ * given: standard gauge, constant eta, BSSN, Kreiss-Oliger dissipation;
+ * kappa = (0.02, 0, 0.5). */
 void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision);
 
 /* One ghost-padded block: the per-block arguments of the reference

@@ -202,6 +202,95 @@ void kreiss_oliger(double *D, const double *u, double h, const Grid &g, unsigned
             for_box(g, b, axis, ie - 3 + m, [&](std::ptrdiff_t p, int, int, int) { D[p] = apply(KO_R[m], u, p, s) / sr[m]; });
 }
 
+// ---- TVB dissipation, bssn/src/derivs.cpp:1348-2054 (disstvb3_{x,y,z},
+// disstvb5_{x,y,z}), weighted by the characteristic speeds of
+// bssn/src/rhs.cpp:984-1028 (max_spacetime_speeds).  Each tap of an interior
+// form is multiplied by the speed along the sweep axis at that tap (TVB3), or
+// each difference u[o+1] - u[o] by the larger of the two speeds (TVB5).
+// Faces flagged as domain boundaries take the Kreiss-Oliger one-sided
+// closures (KO_L / KO_R: the reference writes the same formulas).  Where the
+// reference's result is undefined this restatement makes a choice:
+//   - the speeds are evaluated at every node, ghost nodes included (the
+//     reference fills interior nodes only, and its stencils read ghost ones);
+//   - TVB3 at the first and last interior node before an unflagged face uses
+//     the interior form (the reference leaves the array's previous contents).
+// The reference scales the TVB5 interior form by 2/(75 h) in x and y and by
+// -1/(64 h) in z (derivs.cpp:1668, 1796, 1929); kept as it is.
+const Stencil TVB3_INT = {5, {{-2, 1.0}, {-1, -4.0}, {0, 6.0}, {1, -4.0}, {2, 1.0}}};
+const Stencil TVB5_FIRST = {6, {{3, 1.0}, {2, -6.0}, {1, 15.0}, {0, -19.0}, {-1, 12.0}, {-2, -3.0}}};  // / smr1
+const Stencil TVB5_LAST = {6, {{-4, 1.0}, {-3, -6.0}, {-2, 15.0}, {-1, -19.0}, {0, 12.0}, {1, -3.0}}};  // / spr1
+const Tap TVB5_DIFF[6] = {{2, 1.0}, {1, -5.0}, {0, 10.0}, {-1, -10.0}, {-2, 5.0}, {-3, -1.0}};
+
+// sum of (c lam) u over the taps, left to right (the reference's order)
+double apply_w(const Stencil &s, const double *u, const double *lam, std::ptrdiff_t p, std::ptrdiff_t stride)
+{
+    std::ptrdiff_t q = p + s.t[0].off * stride;
+    double acc = s.t[0].c * lam[q] * u[q];
+    for (int m = 1; m < s.n; m++) {
+        q = p + s.t[m].off * stride;
+        acc = acc + s.t[m].c * lam[q] * u[q];
+    }
+    return acc;
+}
+
+// std::max(a, b)
+double larger(double a, double b) { return a < b ? b : a; }
+
+double tvb5_diff(const double *u, const double *lam, std::ptrdiff_t p, std::ptrdiff_t s)
+{
+    double acc = 0.0;
+    for (int m = 0; m < 6; m++) {
+        const std::ptrdiff_t q0 = p + TVB5_DIFF[m].off * s, q1 = q0 + s;
+        const double t = TVB5_DIFF[m].c * larger(lam[q1], lam[q0]) * (u[q1] - u[q0]);
+        acc = m ? acc + t : t;
+    }
+    return acc;
+}
+
+void tvb(double *D, const double *u, const double *lam, double h, const Grid &g, unsigned bflag, int axis, int order)
+{
+    const double smr3 = 59.0 / 48.0 * 64 * h;
+    const double smr2 = 43.0 / 48.0 * 64 * h;
+    const double smr1 = 49.0 / 48.0 * 64 * h;
+    const double sl[3] = {smr3, smr2, smr1};
+    const double sr[3] = {smr1, smr2, smr3};
+    const Box b = interior(g);
+    const std::ptrdiff_t s = g.stride(axis);
+    const int ie = g.n(axis) - 3;
+    if (order == 3) {
+        const double pre = -1.0 / 12.0 / h;
+        for_box(g, b, axis, -1, [&](std::ptrdiff_t p, int, int, int) { D[p] = pre * apply_w(TVB3_INT, u, lam, p, s); });
+    } else {
+        const double pre = axis == 2 ? -1.0 / 64.0 / h : 2.0 / 75.0 / h;
+        for_box(g, b, axis, -1, [&](std::ptrdiff_t p, int, int, int) { D[p] = pre * tvb5_diff(u, lam, p, s); });
+        for_box(g, b, axis, 3, [&](std::ptrdiff_t p, int, int, int) { D[p] = apply_w(TVB5_FIRST, u, lam, p, s) / smr1; });
+        for_box(g, b, axis, ie - 1, [&](std::ptrdiff_t p, int, int, int) { D[p] = apply_w(TVB5_LAST, u, lam, p, s) / smr1; });
+    }
+    if (bflag & (1u << (2 * axis)))
+        for (int m = 0; m < 3; m++)
+            for_box(g, b, axis, 3 + m, [&](std::ptrdiff_t p, int, int, int) { D[p] = apply(KO_L[m], u, p, s) / sl[m]; });
+    if (bflag & (1u << (2 * axis + 1)))
+        for (int m = 0; m < 3; m++)
+            for_box(g, b, axis, ie - 3 + m, [&](std::ptrdiff_t p, int, int, int) { D[p] = apply(KO_R[m], u, p, s) / sr[m]; });
+}
+
+// Characteristic speeds at every node of a block (rhs.cpp:984-1028):
+// lam_d = max(|-beta^d + t_d|, |-beta^d - t_d|), t_d = alpha sqrt(gtu_dd chi),
+// gtu_dd the diagonal cofactors of gt (det gt = 1 assumed), all three set to 1
+// where one of them is negative.
+void spacetime_speeds(double *const lam[3], const double *const *in, std::size_t n)
+{
+    for (std::size_t p = 0; p < n; p++) {
+        double u11 = in[15][p] * in[17][p] - in[16][p] * in[16][p];
+        double u22 = in[12][p] * in[17][p] - in[14][p] * in[14][p];
+        double u33 = in[12][p] * in[15][p] - in[13][p] * in[13][p];
+        if (u11 < 0.0 || u22 < 0.0 || u33 < 0.0) u11 = u22 = u33 = 1.0;
+        const double t[3] = {in[0][p] * std::sqrt(u11 * in[1][p]), in[0][p] * std::sqrt(u22 * in[1][p]),
+                             in[0][p] * std::sqrt(u33 * in[1][p])};
+        for (int d = 0; d < 3; d++) lam[d][p] = larger(std::fabs(-in[6 + d][p] + t[d]), std::fabs(-in[6 + d][p] - t[d]));
+    }
+}
+
 // Fields with second derivatives (bssn/scripts/bssnrhs_derivs.h): alpha,
 // beta0..2, chi, gt0..5.
 const int D2_FIELDS[11] = {0, 6, 7, 8, 1, 12, 13, 14, 15, 16, 17};
@@ -303,6 +392,7 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
 {
     if (!outv || !inv || !pmin || !pmax || !sz || !P) return -1;
     if (sz[0] < 13 || sz[1] < 13 || sz[2] < 13 || bflag > 63) return -2;
+    if (P->dissipation < 0 || P->dissipation > 2 || (P->dissipation && P->ccz4)) return -3;
     const Grid g{(int)sz[0], (int)sz[1], (int)sz[2]};
     const std::size_t n = (std::size_t)g.nx * g.ny * g.nz;
     const int nf = P->ccz4 ? 25 : 24;
@@ -392,11 +482,23 @@ int rhs_block(double **outv, const double **inv, uint64_t offset, const double *
         }
     }
 
-    // Kreiss-Oliger sweeps into the first-derivative arrays, then
-    // rhs += sigma (KO_x + KO_y + KO_z) on the interior (rhs.cpp:324-369).
+    // Kreiss-Oliger (or TVB, rhs.cpp:729-757) sweeps into the first-derivative
+    // arrays, then rhs += sigma (D_x + D_y + D_z) on the interior (rhs.cpp:324-369).
     const double sigma = P->ko_sigma;
+    std::vector<double> lamv;
+    double *lam[3] = {nullptr, nullptr, nullptr};
+    if (P->dissipation) {
+        lamv.assign(3 * n, 0.0);
+        for (int d = 0; d < 3; d++) lam[d] = lamv.data() + d * n;
+        spacetime_speeds(lam, in, n);
+    }
     for (int f = 0; f < nf; f++) {
-        for (int a = 0; a < 3; a++) kreiss_oliger(W.grad[f][a], in[f], h[a], g, bflag, a);
+        for (int a = 0; a < 3; a++) {
+            if (P->dissipation)
+                tvb(W.grad[f][a], in[f], lam[a], h[a], g, bflag, a, P->dissipation == 1 ? 3 : 5);
+            else
+                kreiss_oliger(W.grad[f][a], in[f], h[a], g, bflag, a);
+        }
         for_box(g, interior(g), 0, -1, [&](std::ptrdiff_t pp, int, int, int) {
             out[f][pp] += sigma * (W.grad[f][0][pp] + W.grad[f][1][pp] + W.grad[f][2][pp]);
         });
@@ -492,7 +594,25 @@ extern "C" int oracle_deriv(int which, double *out, const double *u, double h, c
         }
         break;
     case 3: kreiss_oliger(out, u, h, g, bflag, a); break;
+    case 4:
+    case 5:
+        if (!beta) return -1;
+        tvb(out, u, beta, h, g, bflag, a, which / 3 == 4 ? 3 : 5);
+        break;
     default: return -1;
     }
+    return 0;
+}
+
+extern "C" int oracle_spacetime_speeds(double **lam, const double **inv, uint64_t offset, const uint32_t *sz)
+{
+    if (!lam || !inv || !sz || !lam[0] || !lam[1] || !lam[2]) return -1;
+    const double *in[24];
+    for (int f = 0; f < 24; f++) {
+        if (!inv[f]) return -1;
+        in[f] = inv[f] + offset;
+    }
+    double *const l[3] = {lam[0] + offset, lam[1] + offset, lam[2] + offset};
+    spacetime_speeds(l, in, (std::size_t)sz[0] * sz[1] * sz[2]);
     return 0;
 }

@@ -33,6 +33,8 @@ typedef struct {
     int32_t ccz4;            /* 1: the CCZ4 system (sympygr_amd/codegen/ccz4_eqns.py):
                                 25 fields, Theta last; the arrays hold 25 pointers */
     double kappa[3];         /* CCZ4 damping kappa1..3 */
+    int32_t dissipation;     /* DISSIPATION_TYPE (rhs.cpp:729-757): 0 Kreiss-Oliger,
+                                1 TVB3, 2 TVB5 (BSSN only) */
 } oracle_params;
 
 /* One block, same argument meaning as the reference bssnrhs (rhs.h:33-36). */
@@ -61,11 +63,16 @@ int oracle_ghost_fill(double **u, uint32_t nblocks, const uint64_t *offsets, con
                       const double *pmin, const double *pmax, const uint32_t *bflag);
 
 /* One stencil sweep: which = 0..2 first x/y/z, 3..5 second xx/yy/zz,
- * 6..8 advective x/y/z (needs beta_d), 9..11 Kreiss-Oliger x/y/z.
- * upwind selects the advective stencil as in oracle_params. */
+ * 6..8 advective x/y/z (needs beta_d), 9..11 Kreiss-Oliger x/y/z,
+ * 12..14 TVB3 x/y/z and 15..17 TVB5 x/y/z (beta holds the speeds along the
+ * axis).  upwind selects the advective stencil as in oracle_params. */
 int oracle_deriv(int which, double *out, const double *u, double h,
                  const uint32_t *sz, const double *beta, uint32_t bflag,
                  int upwind);
+
+/* Characteristic speeds of the TVB dissipation (rhs.cpp:984-1028) at every
+ * node of one block, ghost nodes included: lam[d][offset + p]. */
+int oracle_spacetime_speeds(double **lam, const double **in, uint64_t offset, const uint32_t *sz);
 
 #ifdef __cplusplus
 }

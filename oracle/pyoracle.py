@@ -30,13 +30,17 @@ class OracleParams(ctypes.Structure):
         ("eta_power", ctypes.c_double * 2),
         ("ccz4", ctypes.c_int32),
         ("kappa", ctypes.c_double * 3),
+        ("dissipation", ctypes.c_int32),
     ]
+
+DISSIPATION = {"ko": 0, "tvb3": 1, "tvb5": 2}
 
 
 def make_params(upwind=True, ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
                 lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
-                eta_func_r0=1.31, eta_power=(2.0, 2.0), ccz4=False, kappa=(0.02, 0.0, 0.5)):
+                eta_func_r0=1.31, eta_power=(2.0, 2.0), ccz4=False, kappa=(0.02, 0.0, 0.5), dissipation="ko"):
     p = OracleParams()
+    p.dissipation = DISSIPATION[dissipation]
     p.ccz4 = 1 if ccz4 else 0
     for i in range(3):
         p.kappa[i] = kappa[i]
@@ -78,6 +82,8 @@ def lib():
                                         ctypes.POINTER(ctypes.c_uint32), P_D, P_D, ctypes.POINTER(ctypes.c_uint32)]
         L.oracle_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), P_D,
                                    ctypes.c_uint32, ctypes.c_int]
+        L.oracle_spacetime_speeds.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64,
+                                              ctypes.POINTER(ctypes.c_uint32)]
         _lib = L
     return _lib
 
@@ -136,6 +142,15 @@ def deriv(which, u, h, sz, bflag, beta=None, upwind=True):
     return out
 
 
+def spacetime_speeds(u, sz):
+    """(3, npts): the TVB characteristic speeds at every node of one block."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.zeros((3, u.shape[1]))
+    if lib().oracle_spacetime_speeds(_ptrs(lam), _ptrs(u), 0, (ctypes.c_uint32 * 3)(*sz)) != 0:
+        raise RuntimeError("oracle_spacetime_speeds failed")
+    return lam
+
+
 # ---------------------------------------------------------------- reference
 def ref_path(variant):
     return os.path.join(HERE, "_ref", f"libbssnref_{variant}.so")
@@ -159,6 +174,10 @@ def ref_lib(variant):
         L.ref_fake_initial_data.argtypes = [ctypes.c_double] * 3 + [P_D]
         L.ref_bssnrhs_sep.argtypes = L.ref_bssnrhs.argtypes
         L.ref_set_gauge_params.argtypes = [ctypes.c_double, P_D, ctypes.POINTER(ctypes.c_uint)]
+        L.ref_set_dissipation.argtypes = [ctypes.c_uint]
+        L.ref_tvb.argtypes = [ctypes.c_int, P_D, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
+                              ctypes.c_uint]
+        L.ref_spacetime_speeds.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.POINTER(ctypes.c_uint)]
         _ref[variant] = L
     return _ref[variant]
 
@@ -175,9 +194,12 @@ def ref_rhs_block(variant, u, sz, pmin, pmax, bflag, params, staged=False):
     L.ref_set_gauge_params(params.eta_func_r0, (ctypes.c_double * 2)(*params.eta_power),
                            (ctypes.c_uint * 3)(*params.xi))
     fn = L.ref_bssnrhs_sep if staged else L.ref_bssnrhs
+    # DISSIPATION_TYPE is read by bssnrhs_sep only (rhs.cpp:729-757)
+    L.ref_set_dissipation(params.dissipation if staged else 0)
     fn(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
        (ctypes.c_uint * 3)(*sz), bflag, lam, lf, params.eta_const, params.eta_r0,
        params.eta_damping_exp, params.ko_sigma)
+    L.ref_set_dissipation(0)
     return out
 
 
@@ -215,6 +237,29 @@ def ref_deriv(variant, which, u, h, sz, bflag, beta=None):
     b = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64).ctypes.data_as(P_D)
     L.ref_deriv(which, out.ctypes.data_as(P_D), u.ctypes.data_as(P_D), h, (ctypes.c_uint * 3)(*sz), b, bflag)
     return out
+
+
+def ref_tvb(variant, which, u, lam, h, sz, bflag):
+    """One reference TVB sweep: which 0..2 disstvb3_{x,y,z}, 3..5
+    disstvb5_{x,y,z}, with the speeds `lam` along the axis.  Nodes the sweep
+    does not write stay NaN."""
+    L = ref_lib(variant)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.full_like(u, np.nan)
+    L.ref_tvb(which, out.ctypes.data_as(P_D), u.ctypes.data_as(P_D), lam.ctypes.data_as(P_D), h,
+              (ctypes.c_uint * 3)(*sz), bflag)
+    return out
+
+
+def ref_spacetime_speeds(variant, u, sz):
+    """(3, npts): the reference's max_spacetime_speeds; nodes it does not
+    write (ghost nodes) stay NaN."""
+    L = ref_lib(variant)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.full((3, u.shape[1]), np.nan)
+    L.ref_spacetime_speeds(_ptrs(lam), _ptrs(u), (ctypes.c_uint * 3)(*sz))
+    return lam
 
 
 def ref_fake_initial_data(variant, sz, pmin, pmax):

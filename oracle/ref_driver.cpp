@@ -94,6 +94,34 @@ void ref_set_gauge_params(double eta_func_r0, const double *eta_power, const uns
     for (int i = 0; i < 3; i++) bssn::BSSN_XI[i] = xi[i];
 }
 
+// DISSIPATION_TYPE (bssn/src/parameters.cpp:109), read by bssnrhs_sep
+// (bssn/src/rhs.cpp:729-757): 0 Kreiss-Oliger, 1 TVB3, 2 TVB5.
+void ref_set_dissipation(unsigned int type) { bssn::DISSIPATION_TYPE = type; }
+
+// One TVB sweep (bssn/src/derivs.cpp:1348-2054): which 0..2 disstvb3_{x,y,z},
+// 3..5 disstvb5_{x,y,z}; lam is the speed along the sweep axis.
+int ref_tvb(int which, double *out, const double *u, const double *lam, double h, const unsigned int *sz,
+            unsigned int bflag)
+{
+    switch (which) {
+    case 0: disstvb3_x(out, u, lam, h, sz, bflag); break;
+    case 1: disstvb3_y(out, u, lam, h, sz, bflag); break;
+    case 2: disstvb3_z(out, u, lam, h, sz, bflag); break;
+    case 3: disstvb5_x(out, u, lam, h, sz, bflag); break;
+    case 4: disstvb5_y(out, u, lam, h, sz, bflag); break;
+    case 5: disstvb5_z(out, u, lam, h, sz, bflag); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+// The TVB characteristic speeds of one block (bssn/src/rhs.cpp:984-1028).
+void ref_spacetime_speeds(double **lam, const double **in, const unsigned int *sz)
+{
+    max_spacetime_speeds(lam[0], lam[1], lam[2], in[0], in[6], in[7], in[8], in[12], in[13], in[14], in[15], in[16],
+                         in[17], in[1], sz);
+}
+
 // Single stencil sweeps (bssn/src/derivs.cpp), selected by a small integer:
 // 0..2 deriv_{x,y,z}, 3..5 deriv_{xx,yy,zz}, 6..8 adv_deriv_{x,y,z},
 // 9..11 ko_deriv_{x,y,z}.  `beta` is only read by the advective sweeps.

@@ -49,6 +49,7 @@ class SgrParams(ctypes.Structure):
         ("eta_power", ctypes.c_double * 2),
         ("formulation", ctypes.c_int32),
         ("kappa", ctypes.c_double * 3),
+        ("dissipation", ctypes.c_int32),
     ]
 
 
@@ -150,19 +151,23 @@ def check(rc):
 GAUGE = {"standard": 0, "rochester": 1}
 ETA = {"const": 0, "func": 1}
 FORM = {"bssn": 0, "ccz4": 1}
+DISS = {"ko": 0, "tvb3": 1, "tvb5": 2}
 NUM_VARS_CCZ4 = 25
 VAR_NAMES_CCZ4 = VAR_NAMES + ["Theta"]
 
 
 def params(adv="upwind", precision="fp64", ko_sigma=0.1, eta_const=2.0, eta_r0=50.0, eta_damping_exp=1.0,
            lambda_=(1, 1, 1, 1), lambda_f=(1.0, 0.0), gauge="standard", eta="const", xi=(0, 0, 0),
-           eta_func_r0=1.31, eta_power=(2.0, 2.0), formulation="bssn", kappa=(0.02, 0.0, 0.5)) -> SgrParams:
+           eta_func_r0=1.31, eta_power=(2.0, 2.0), formulation="bssn", kappa=(0.02, 0.0, 0.5),
+           dissipation="ko") -> SgrParams:
     """sgr_params: the reference's parameter defaults with the given
     choices.  formulation "ccz4" evolves the CCZ4 system (25 fields, Theta
-    last) with damping kappa = (kappa1, kappa2, kappa3)."""
+    last) with damping kappa = (kappa1, kappa2, kappa3).  dissipation "ko"
+    (Kreiss-Oliger), "tvb3" or "tvb5" (DISSIPATION_TYPE 0, 1, 2)."""
     p = SgrParams()
     lib().sgr_params_default(ctypes.byref(p), ADV[adv], PREC[precision])
     p.formulation = FORM[formulation]
+    p.dissipation = DISS[dissipation]
     for i in range(3):
         p.kappa[i] = kappa[i]
     for i in range(4):

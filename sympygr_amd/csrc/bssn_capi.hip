@@ -179,6 +179,10 @@ int validate_params(const sgr_params *p)
         return fail(SGR_E_INVALID, "formulation must be SGR_FORM_BSSN or SGR_FORM_CCZ4");
     if (p->formulation == SGR_FORM_CCZ4 && p->gauge != SGR_GAUGE_STANDARD)
         return fail(SGR_E_UNSUPPORTED, "CCZ4 is evolved with the standard gauge only");
+    if (p->dissipation != SGR_DISS_KO && p->dissipation != SGR_DISS_TVB3 && p->dissipation != SGR_DISS_TVB5)
+        return fail(SGR_E_INVALID, "dissipation must be SGR_DISS_KO, SGR_DISS_TVB3 or SGR_DISS_TVB5");
+    if (p->formulation == SGR_FORM_CCZ4 && p->dissipation != SGR_DISS_KO)
+        return fail(SGR_E_UNSUPPORTED, "TVB dissipation is defined for the BSSN system only");
     for (int i = 0; i < 3; i++)
         if (!std::isfinite(p->kappa[i])) return fail(SGR_E_INVALID, "kappa must be finite");
     return SGR_OK;
@@ -267,6 +271,10 @@ struct sgr_plan {
     //                their NF double KO sums per point
     void *d_ws = nullptr;
     int *d_flag = nullptr;
+    // TVB dissipation: characteristic speeds along x, y, z at every node of
+    // the field arrays (lam_span nodes each, the end of the last block)
+    double *d_lam = nullptr;
+    uint64_t lam_span = 0;
     uint32_t chunk_tiles = 0;    // tiles per RHS launch (workspace capacity)
     uint32_t cchunk_tiles = 0;   // tiles per constraint launch
     uint64_t ws_bytes = 0;
@@ -291,6 +299,7 @@ static void plan_free(sgr_plan *p)
     hipFree(p->d_queue);
     hipFree(p->d_ws);
     hipFree(p->d_flag);
+    hipFree(p->d_lam);
     hipFree(p->d_ghost);
     hipFree(p->d_gbox);
     hipSetDevice(cur);
@@ -321,6 +330,7 @@ void sgr_params_default(sgr_params *p, int32_t adv_mode, int32_t precision)
     p->kappa[0] = 0.02;
     p->kappa[1] = 0.0;
     p->kappa[2] = 0.5;
+    p->dissipation = SGR_DISS_KO;
 }
 
 int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, const sgr_params *params, int device)
@@ -355,6 +365,9 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         }
         p->hblocks.push_back(block_desc(blocks[b], &shape[3 * b], 0, 0));
         p->interior += (uint64_t)(blocks[b].sz[0] - 6) * (blocks[b].sz[1] - 6) * (blocks[b].sz[2] - 6);
+        if (params->dissipation != SGR_DISS_KO)
+            p->lam_span = std::max<uint64_t>(p->lam_span, blocks[b].offset + (uint64_t)blocks[b].sz[0] *
+                                                                               blocks[b].sz[1] * blocks[b].sz[2]);
     }
     for (int pass = 0; pass < 2; pass++) {
         for (uint32_t b = 0; b < nblocks; b++) {
@@ -413,7 +426,8 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     {
         const char *lv = getenv("SGR_DEVICE_MEM_LIMIT_GIB");
         const uint64_t need = p->ws_bytes + nblocks * sizeof(BlockDesc) +
-                              (tile_block.size() + tile_block8.size()) * sizeof(uint32_t);
+                              (tile_block.size() + tile_block8.size()) * sizeof(uint32_t) +
+                              3 * p->lam_span * sizeof(double);
         if (lv && (double)need > atof(lv) * 1073741824.0) {
             char msg[160];
             snprintf(msg, sizeof msg, "plan needs %.3f GiB of device memory, over SGR_DEVICE_MEM_LIMIT_GIB=%s",
@@ -468,6 +482,9 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     }
     if ((e = dev_alloc((void **)&p->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc(flag)");
     if ((e = dev_alloc((void **)&p->d_queue, 8 * sizeof(uint32_t))) != hipSuccess) return bail(e, "hipMalloc(queue)");
+    if (p->lam_span &&
+        (e = dev_alloc((void **)&p->d_lam, 3 * p->lam_span * sizeof(double))) != hipSuccess)
+        return bail(e, "hipMalloc(speeds)");
     p->grid_cap = resident_stencil_groups(device);
     p->ghost = build_ghost_table(blocks, nblocks);
     if (!p->ghost.patches.empty()) {
@@ -529,13 +546,22 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     for (int q = 0; q < 4; q++) a.ev[q] = ev ? ev[q] : nullptr;
     a.rk.stage = -1;
     if (rk) a.rk = *rk;
+    if (plan->params.dissipation != SGR_DISS_KO) {
+        // TVB replaces the Kreiss-Oliger term: the stencil pass leaves it out
+        // of the tails and k_tvb adds sigma (D_x + D_y + D_z) to them
+        a.diss = plan->params.dissipation;
+        a.diss_sigma = plan->params.ko_sigma;
+        a.params.ko_sigma = 0.0;
+        a.lam = plan->d_lam;
+        a.lam_span = plan->lam_span;
+    }
     int cur = 0;
     hipGetDevice(&cur);
     if (cur != plan->device) HIP_TRY(hipSetDevice(plan->device), "hipSetDevice");
-    hipError_t e = hipSuccess;
-    if (plan->ntiles <= plan->chunk_tiles) {
+    hipError_t e = a.diss ? launch_speeds(a, stream) : hipSuccess;  // once, over every node
+    if (e == hipSuccess && plan->ntiles <= plan->chunk_tiles) {
         e = launch_rhs(a, stream);
-    } else {
+    } else if (e == hipSuccess) {
         // several workspace chunks: events[0..1] bracket the whole call,
         // events[2..3] are recorded at its end
         if (ev) e = hipEventRecord(ev[0], stream);
@@ -837,7 +863,8 @@ int stage_plan(const sgr_block &b, const sgr_params *params, int device)
     }
     const bool same = S.plan && S.sz[0] == b.sz[0] && S.sz[1] == b.sz[1] && S.sz[2] == b.sz[2] &&
                       S.adv == params->adv_mode && S.prec == params->precision &&
-                      S.plan->params.formulation == params->formulation;
+                      S.plan->params.formulation == params->formulation &&
+                      S.plan->params.dissipation == params->dissipation;  // TVB plans hold the speed arrays
     if (!same) {
         if (S.plan) {
             hipStreamSynchronize(S.stream);
@@ -1037,6 +1064,8 @@ int sgr_rhs_block_noalloc(double *const out[SGR_NUM_VARS], const double *const i
     sgr_block b;
     int rc = block_checks(pmin, pmax, sz, bflag, params, offset, b);
     if (rc) return rc;
+    if (params->dissipation != SGR_DISS_KO)
+        return fail(SGR_E_UNSUPPORTED, "TVB dissipation needs the speed arrays of a plan (sgr_plan_create)");
     const int nf = params->formulation == SGR_FORM_CCZ4 ? NF_CCZ4 : NF;
     for (int f = 0; f < nf; f++)
         if (!in[f] || !out[f]) return fail(SGR_E_INVALID, "null field pointer");

@@ -39,6 +39,13 @@ struct LaunchArgs {
     hipEvent_t ev[4];            // optional: start/stop of k_stencil, start/stop of k_pointwise
     RkArgs rk;                   // rk.stage < 0: plain RHS
     int derivs_threads;          // constraint pass: k_derivs workgroup size, 256 or 512
+    // RHS dissipation: 0 Kreiss-Oliger (in the stencil pass, params.ko_sigma),
+    // 1 TVB3, 2 TVB5 (k_tvb adds diss_sigma times it to the tails; the
+    // stencil pass then runs with params.ko_sigma = 0)
+    int diss;
+    double diss_sigma;
+    double *lam;                 // TVB: speeds along x, y, z, lam_span nodes each
+    uint64_t lam_span;
 };
 
 // Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints; uses the workspace.
@@ -49,5 +56,8 @@ hipError_t launch_ghost_fill(OutPtrs u, int nf, const GhostPatch *patches, const
                              const uint32_t bstart[GHOST_CLASSES + 1], hipStream_t s);
 
 hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s);    // derivative workspace + pointwise
+
+// TVB dissipation: the characteristic speeds at every node (before launch_rhs)
+hipError_t launch_speeds(const LaunchArgs &a, hipStream_t s);
 
 }  // namespace sgr

@@ -812,6 +812,92 @@ __global__ __launch_bounds__(512, 4) void k_stencil(FieldPtrs in, const BlockDes
 #undef SGR_PW_ALG
 #undef SGR_PW_NFLD
 
+// ------------------------------------------------------------ TVB dissipation
+// Characteristic speeds at every node of the field arrays, ghost nodes
+// included (bssn/src/rhs.cpp:984-1028 fills interior nodes only):
+// lam[d span + p] = max(|-beta^d + t|, |-beta^d - t|), t = alpha sqrt(gtu_dd chi),
+// gtu_dd the diagonal cofactors of gt (det gt = 1 assumed), all three 1
+// where one is negative.  Nodes between blocks get values nothing reads.
+__global__ __launch_bounds__(256) void k_speeds(FieldPtrs in, double *__restrict__ lam, uint64_t span)
+{
+    SGR_EXACT
+    const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (p >= span) return;
+    const double g0 = in.p[12][p], g1 = in.p[13][p], g2 = in.p[14][p];
+    const double g3 = in.p[15][p], g4 = in.p[16][p], g5 = in.p[17][p];
+    double c[3] = {g3 * g5 - g4 * g4, g0 * g5 - g2 * g2, g0 * g3 - g1 * g1};
+    if (c[0] < 0.0 || c[1] < 0.0 || c[2] < 0.0) c[0] = c[1] = c[2] = 1.0;
+    const double alpha = in.p[0][p], chi = in.p[1][p];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double t = alpha * sqrt(c[d] * chi), b = in.p[6 + d][p];
+        lam[d * span + p] = larger(fabs(-b + t), fabs(-b - t));
+    }
+}
+
+// Adds sigma (D_x + D_y + D_z) of TVB3 or TVB5 to the tail of every field at
+// every interior point the launch's tiles own (after the stencil pass, which
+// left the Kreiss-Oliger term out; the reference adds the dissipation after
+// the boundary conditions, rhs.cpp:766-802, so face points get it too).  Same
+// tile -> thread map as the pointwise kernel.
+template <bool TVB5>
+__global__ __launch_bounds__(256) void k_tvb(FieldPtrs in, const double *__restrict__ lam, uint64_t span,
+                                             const BlockDesc *__restrict__ blocks,
+                                             const uint32_t *__restrict__ tile_block, double *__restrict__ wsT,
+                                             double sigma, uint32_t tile0)
+{
+    const uint32_t wg = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tile = tile0 + (wg >> 1);
+    const int l = (int)((wg & 1) * 256 + threadIdx.x);
+    const BlockDesc &B = blocks[tile_block[tile]];
+    const TileCoord tc = tile_coord(B, tile);
+    const int TX = B.tdim[0], TY = B.tdim[1];
+    if (l >= TX * TY * B.tdim[2]) return;
+    const int r = div_q(l, make_div(TX)), lz = div_q(r, make_div(TY));
+    const int pos[3] = {tc.i0 + l - r * TX, tc.j0 + r - lz * TY, tc.k0 + lz};
+    const int nn[3] = {(int)B.n[0], (int)B.n[1], (int)B.n[2]};
+    if (pos[0] >= nn[0] - GW || pos[1] >= nn[1] - GW || pos[2] >= nn[2] - GW) return;
+    if (pos[0] < tc.oi || pos[1] < tc.oj || pos[2] < tc.ok) return;  // owned by the previous tile
+    const int st[3] = {1, nn[0], nn[0] * nn[1]};
+    const uint64_t pp = B.offset + pos[0] + (uint64_t)nn[0] * (pos[1] + (uint64_t)nn[1] * pos[2]);
+    // the speeds at the taps the point's forms read (TVB3: -2..2, TVB5: -4..3)
+    constexpr int OLO = TVB5 ? -4 : -2, OHI = TVB5 ? 3 : 2;
+    double L[3][8], pre[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        pre[d] = TVB5 ? (d == 2 ? B.ax[2].ko_pre : 2.0 / 75.0 / B.ax[d].h) : -1.0 / 12.0 / B.ax[d].h;
+#pragma unroll
+        for (int o = OLO; o <= OHI; o++) {
+            const int q = pos[d] + o;
+            L[d][o + 4] = q >= 0 && q < nn[d] ? lam[d * span + pp + (int64_t)o * st[d]] : 0.0;
+        }
+    }
+    const uint32_t bf = B.bflag;
+    double *__restrict__ tb = wsT + ws_tile_base(wg >> 1, NF) + l;
+    constexpr uint64_t SS = WS_SS;
+#pragma unroll 1
+    for (int f = 0; f < NF; f++) {
+        const double *u = in.p[f] + pp;
+        double D[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            D[d] = tvb<TVB5>(u, st[d], pos[d], nn[d], bf & (1u << (2 * d)), bf & (2u << (2 * d)), L[d], pre[d], B.ax[d]);
+        {
+            SGR_EXACT
+            tb[rs_tail(f) * SS] = tb[rs_tail(f) * SS] + sigma * ((D[0] + D[1]) + D[2]);
+        }
+    }
+}
+
+hipError_t launch_speeds(const LaunchArgs &a, hipStream_t s)
+{
+    if (!a.lam_span) return hipSuccess;
+    const uint64_t g = (a.lam_span + 255) / 256;
+    if (g > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_speeds, dim3((uint32_t)g), dim3(256), 0, s, a.in, a.lam, a.lam_span);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ launch
 __global__ void k_queue_reset(uint32_t *queue)
 {
@@ -881,6 +967,15 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     }
     hipError_t e = launch_stencil<W, UPWIND, NF>(a, ws, s);
     if (e != hipSuccess) return e;
+    if (a.diss) {  // TVB into the tails (BSSN only)
+        if (a.diss == 2)
+            hipLaunchKernelGGL(k_tvb<true>, g, b, 0, s, a.in, a.lam, a.lam_span, a.blocks, a.tile_block, a.wsT,
+                               a.diss_sigma, a.tile0);
+        else
+            hipLaunchKernelGGL(k_tvb<false>, g, b, 0, s, a.in, a.lam, a.lam_span, a.blocks, a.tile_block, a.wsT,
+                               a.diss_sigma, a.tile0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     hipExtLaunchKernelGGL((k_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                           a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
     return hipGetLastError();

@@ -23,6 +23,8 @@ __attribute__((weak)) double KO_DISS_SIGMA = 0.01;
 __attribute__((weak)) double BSSN_ETA_R0 = 1.31;
 __attribute__((weak)) double BSSN_ETA_POWER[2] = {2.0, 2.0};
 __attribute__((weak)) unsigned int BSSN_XI[3] = {0, 0, 0};
+// dissipation of bssnrhs_sep (bssn/src/parameters.cpp:109, rhs.cpp:729-757)
+__attribute__((weak)) unsigned int DISSIPATION_TYPE = 0;
 }  // namespace bssn
 
 namespace {
@@ -69,13 +71,17 @@ sgr_params current_params()
     p.eta_power[1] = bssn::BSSN_ETA_POWER[1];
     p.formulation = SGR_FORM_BSSN;  // the reference's bssnrhs evolves BSSN
     p.kappa[0] = p.kappa[1] = p.kappa[2] = 0.0;
+    p.dissipation = SGR_DISS_KO;    // bssnrhs; bssnrhs_sep reads DISSIPATION_TYPE
     return p;
 }
 
 void run(double **uzipVarsRHS, const double **uZipVars, unsigned int offset, const double *ptmin,
-         const double *ptmax, const unsigned int *sz, unsigned int bflag, const char *who)
+         const double *ptmax, const unsigned int *sz, unsigned int bflag, const char *who, bool sep)
 {
-    const sgr_params p = current_params();
+    sgr_params p = current_params();
+    // a DISSIPATION_TYPE other than 0, 1, 2 fails the call (SGR_E_INVALID);
+    // the reference prints "Unknown DISSIPATION_TYPE" and adds no dissipation
+    if (sep) p.dissipation = bssn::DISSIPATION_TYPE > 2 ? -1 : (int32_t)bssn::DISSIPATION_TYPE;
     const uint32_t s[3] = {sz[0], sz[1], sz[2]};
     const int rc = sgr_bssnrhs_host(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, s, bflag, &p, g_device);
     if (rc != SGR_OK) {
@@ -134,12 +140,13 @@ extern "C" void sgr_dropin_configure_gauge(int32_t gauge, int32_t eta_mode)
 void bssnrhs(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
              const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
 {
-    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs");
+    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs", false);
 }
 
-// The staged form computes the same right-hand side (bssn/src/rhs.cpp:392-825).
+// The staged form computes the same right-hand side (bssn/src/rhs.cpp:392-825)
+// with the dissipation DISSIPATION_TYPE selects (rhs.cpp:729-757).
 void bssnrhs_sep(double **uzipVarsRHS, const double **uZipVars, const unsigned int &offset, const double *ptmin,
                  const double *ptmax, const unsigned int *sz, const unsigned int &bflag)
 {
-    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs_sep");
+    run(uzipVarsRHS, uZipVars, offset, ptmin, ptmax, sz, bflag, "bssnrhs_sep", true);
 }

@@ -187,4 +187,39 @@ __device__ __forceinline__ double ko(const P *u, int s, int pos, int n, bool lo,
     return ko_int(u, s, a);
 }
 
+// ---------------------------------------------------------------- TVB
+__device__ __forceinline__ double larger(double a, double b) { return a < b ? b : a; }  // std::max(a, b)
+
+// TVB dissipation along one axis at one point (disstvb3_* / disstvb5_*,
+// derivs.cpp:1348-2054).  l[o] (o = -4..3, l = L + 4) is the characteristic
+// speed along the axis at tap o; only the taps the point's form reads need
+// be set.  Points next to a flagged face take the Kreiss-Oliger closures
+// (the reference writes the same formulas).  TVB3 takes its interior form at
+// every other point (the reference leaves the first and last interior node
+// before an unflagged face unwritten); TVB5 has one-sided forms at the first
+// and last interior node.  pre: -1/(12h) for TVB3; 2/(75h) in x and y and
+// -1/(64h) in z for TVB5, as the reference has them.
+template <bool TVB5>
+__device__ __forceinline__ double tvb(const double *u, int s, int pos, int n, bool lo, bool hi, const double *L,
+                                      double pre, const AxisConst &a)
+{
+    SGR_EXACT
+    if ((lo && pos < GW + 3) || (hi && pos >= n - GW - 3)) return ko(u, s, pos, n, lo, hi, a);
+    const double *l = L + 4;
+    if (!TVB5)
+        return pre * (l[-2] * tap(u, s, -2) - 4.0 * l[-1] * tap(u, s, -1) + 6.0 * l[0] * tap(u, s, 0) -
+                      4.0 * l[1] * tap(u, s, 1) + l[2] * tap(u, s, 2));
+    if (pos == GW)
+        return (l[3] * tap(u, s, 3) - 6.0 * l[2] * tap(u, s, 2) + 15.0 * l[1] * tap(u, s, 1) -
+                19.0 * l[0] * tap(u, s, 0) + 12.0 * l[-1] * tap(u, s, -1) - 3.0 * l[-2] * tap(u, s, -2)) / a.smr1;
+    if (pos == n - GW - 1)
+        return (l[-4] * tap(u, s, -4) - 6.0 * l[-3] * tap(u, s, -3) + 15.0 * l[-2] * tap(u, s, -2) -
+                19.0 * l[-1] * tap(u, s, -1) + 12.0 * l[0] * tap(u, s, 0) - 3.0 * l[1] * tap(u, s, 1)) / a.smr1;
+    const double t3 = tap(u, s, 3), t2 = tap(u, s, 2), t1 = tap(u, s, 1), t0 = tap(u, s, 0);
+    const double m1 = tap(u, s, -1), m2 = tap(u, s, -2), m3 = tap(u, s, -3);
+    return pre * (larger(l[3], l[2]) * (t3 - t2) - 5.0 * larger(l[2], l[1]) * (t2 - t1) +
+                  10.0 * larger(l[1], l[0]) * (t1 - t0) - 10.0 * larger(l[0], l[-1]) * (t0 - m1) +
+                  5.0 * larger(l[-1], l[-2]) * (m1 - m2) - larger(l[-2], l[-3]) * (m2 - m3));
+}
+
 }  // namespace sgr

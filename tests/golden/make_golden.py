@@ -7,6 +7,7 @@ Run in a container that has the reference sources and a built oracle/_ref:
 Outputs (all numbers, nothing copied from the reference):
   rhs_<case>.npy       interior RHS, shape (24, n-6, n-6, n-6), float64
   stencils.json        SHA-256 of every single-sweep output (bit-exact checks)
+  tvb.json             SHA-256 of the TVB sweeps and characteristic speeds
   manifest.json        input recipes, parameters, flags, input hashes
 """
 from __future__ import annotations
@@ -58,6 +59,55 @@ def gauge_input():
                                          pmin=tuple(GAUGE["pmin"]), pmax=tuple(GAUGE["pmax"]))
 
 
+def tvb_written(which, n, bflag):
+    """Nodes of a TVB sweep's interior box that the reference writes, (k, j, i):
+    disstvb3 leaves the first and last interior node before an unflagged face
+    as they were (derivs.cpp:1374, 1478, 1585)."""
+    m = np.ones((n - 6,) * 3, bool)
+    if which < 3:
+        ax = which
+        for side in range(2):
+            if not bflag & (1 << (2 * ax + side)):
+                sl = [slice(None)] * 3
+                sl[2 - ax] = 0 if side == 0 else n - 7
+                m[tuple(sl)] = False
+    return m
+
+
+def tvb_sweep_hash(out, which, n, bflag):
+    a = out.reshape(n, n, n)[3:-3, 3:-3, 3:-3]
+    return sha(np.where(tvb_written(which, n, bflag), a, 0.0))
+
+
+def write_tvb_hashes(ug, uf):
+    """tvb.json: speeds of the gauge input at its interior nodes; the six TVB
+    sweeps of the fake input's gt1 (field 13) with the oracle's speeds of the
+    fake input along the sweep axis, h = 0.1, four face-flag sets; and, for the
+    disstvb3 nodes next to unflagged faces, the reference's interior form one
+    node further in on the input shifted by one node along the axis (the value
+    the restatement uses there)."""
+    t = {}
+    n = GAUGE["n"]
+    lam = po.ref_spacetime_speeds("u", ug, (n, n, n))
+    t["speeds_gauge"] = sha(np.stack([lam[d].reshape(n, n, n)[3:-3, 3:-3, 3:-3] for d in range(3)]))
+    n = FAKE["n"]
+    lf = po.spacetime_speeds(uf, (n, n, n))
+    for which in range(6):
+        for bflag in (0, 63, 21, 42):
+            out = po.ref_tvb("u", which, uf[13], lf[which % 3], 0.1, (n, n, n), bflag)
+            t[f"sweep_{which}_{bflag}"] = tvb_sweep_hash(out, which, n, bflag)
+    for ax in range(3):
+        for side, shift, plane in ((0, 1, 4), (1, -1, n - 5)):
+            us = np.roll(uf[13].reshape(n, n, n), shift, axis=2 - ax).ravel()
+            ls = np.roll(lf[ax].reshape(n, n, n), shift, axis=2 - ax).ravel()
+            out = po.ref_tvb("u", ax, us, ls, 0.1, (n, n, n), 0).reshape(n, n, n)
+            sl = [slice(3, -3)] * 3
+            sl[2 - ax] = plane
+            t[f"tvb3_shift_{ax}_{side}"] = sha(out[tuple(sl)])
+    with open(os.path.join(HERE, "tvb.json"), "w") as f:
+        json.dump(t, f, indent=1, sort_keys=True)
+
+
 def main():
     manifest = {"reference_build": REF_FLAGS, "params": PARAMS, "cases": {}}
     u0 = c0_input()
@@ -99,6 +149,20 @@ def main():
         manifest["cases"][name] = dict(input="gauge", variant=var, gauge=gauge, eta=eta,
                                        entry="bssnrhs_sep" if staged else "bssnrhs", bflag=63, n=n,
                                        file=f"rhs_{name}.npy", sha256=sha(out))
+    # TVB dissipation of bssnrhs_sep (DISSIPATION_TYPE 1 and 2) on the gauge
+    # input.  With every face flagged the reference's result is fully defined;
+    # with faces 1, 3, 5 flagged (bflag 42) the nodes within 2 (TVB3) or 3
+    # (TVB5) of an unflagged face read speeds the reference never wrote, and a
+    # test compares the other nodes only.
+    for diss in ("tvb3", "tvb5"):
+        for bflag in (63, 42):
+            name = f"{diss}_staged_b{bflag}"
+            P = po.make_params(upwind=True, dissipation=diss, **PARAMS)
+            out = interior(po.ref_rhs_block("u", ug, (n, n, n), GAUGE["pmin"], GAUGE["pmax"], bflag, P, staged=True), n)
+            np.save(os.path.join(HERE, f"rhs_{name}.npy"), out)
+            manifest["cases"][name] = dict(input="gauge", variant="u", dissipation=diss, entry="bssnrhs_sep",
+                                           bflag=bflag, n=n, file=f"rhs_{name}.npy", sha256=sha(out))
+    write_tvb_hashes(ug, uf)
     # single-sweep hashes on the fake input's gt1 (field 13) with beta from the
     # same input, 13^3 block, h = 0.1
     st = {}
@@ -113,7 +177,7 @@ def main():
         json.dump(st, f, indent=1, sort_keys=True)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
-    print("wrote", len(cases) + len(gcases), "rhs fixtures and", len(st), "stencil hashes")
+    print("wrote", len(cases) + len(gcases) + 4, "rhs fixtures,", len(st), "stencil hashes and tvb.json")
 
 
 if __name__ == "__main__":

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from helpers import GOLDEN, case_gauge, case_input, case_params, golden, interior, manifest, normwise
+from helpers import GOLDEN, case_gauge, case_input, case_params, fixture_mask, golden, interior, manifest, normwise
 from sympygr_amd import data
 
 
@@ -28,7 +28,8 @@ def test_oracle_matches_reference_fixture(name):
     P = po.make_params(upwind=up, ko_sigma=p["ko_sigma"], eta_const=p["eta_const"], eta_r0=p["eta_r0"],
                        eta_damping_exp=p["eta_damping_exp"], **case_gauge(c))
     out = po.rhs_block(u, sz, pmin, pmax, c["bflag"], P)
-    err = normwise(interior(out, sz), golden(name))
+    m = fixture_mask(c)
+    err = normwise(interior(out, sz)[:, m], golden(name)[:, m])
     assert err.max() <= 1e-13, err
 
 
