@@ -116,6 +116,13 @@ typedef struct {
     uint64_t workspace_bytes;    /* device scratch held by the plan */
     uint32_t derivative_slots;   /* derivative values staged per point */
     uint32_t chunks;             /* launches per pass (workspace budget SGR_WS_GIB) */
+    uint32_t pipelined;          /* 1: the chunks run pipelined (two workspace buffers; each
+                                    chunk's stencil pass overlaps the previous chunk's
+                                    pointwise pass on a second stream the plan owns), as
+                                    every plan needing more than one chunk does */
+    uint32_t tile_queue;         /* 1: the stencil pass is a persistent grid fed by per-XCD
+                                    tile queues (plans where at least a tenth of the tiles
+                                    reach a flagged face) */
 } sgr_plan_info;
 
 /* Replaces the per-block setup of the reference bssnRHS block loop

@@ -71,6 +71,8 @@ class SgrPlanInfo(ctypes.Structure):
         ("workspace_bytes", ctypes.c_uint64),
         ("derivative_slots", ctypes.c_uint32),
         ("chunks", ctypes.c_uint32),
+        ("pipelined", ctypes.c_uint32),
+        ("tile_queue", ctypes.c_uint32),
     ]
 
 

@@ -60,6 +60,10 @@ AxisConst axis_const(double pmin, double pmax, uint32_t n)
     return a;
 }
 
+// Chunks of a pipelined plan, at least (the octree of 3032 leaves ran 10-12%
+// faster in FP64 with 16 pipelined chunks than with its 3 consecutive ones).
+constexpr uint64_t PIPE_CHUNKS = 16;
+
 // Workspace bytes per 8^3 tile: RHS pass (staged derivatives in W plus the
 // double tails) and constraint pass (its derivative slots plus KO sums).
 uint64_t rhs_tile_bytes(size_t wsz, int nf)
@@ -275,6 +279,12 @@ struct sgr_plan {
     // the field arrays (lam_span nodes each, the end of the last block)
     double *d_lam = nullptr;
     uint64_t lam_span = 0;
+    // Pipelined schedule (two workspace buffers): the stencil pass of chunk
+    // c + 1 on the caller's stream overlaps the pointwise pass of chunk c on
+    // `side`; pev: fork, join, stencil done (per buffer), pointwise done (per buffer)
+    bool pipe = false;
+    hipStream_t side = nullptr;
+    hipEvent_t pev[6] = {};
     uint32_t chunk_tiles = 0;    // tiles per RHS launch (workspace capacity)
     uint32_t cchunk_tiles = 0;   // tiles per constraint launch
     uint64_t ws_bytes = 0;
@@ -300,6 +310,12 @@ static void plan_free(sgr_plan *p)
     hipFree(p->d_ws);
     hipFree(p->d_flag);
     hipFree(p->d_lam);
+    if (p->side) {
+        hipStreamSynchronize(p->side);
+        hipStreamDestroy(p->side);
+    }
+    for (hipEvent_t &e : p->pev)
+        if (e) hipEventDestroy(e);
     hipFree(p->d_ghost);
     hipFree(p->d_gbox);
     hipSetDevice(cur);
@@ -415,10 +431,21 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         uint64_t cap = (uint64_t)(gib * 1073741824.0) / rhs_tile;
         if (cap < 1) cap = 1;
         const uint64_t nt = tile_block.size();
+        // A plan whose workspace takes more than one chunk runs pipelined:
+        // at least PIPE_CHUNKS chunks (twice as many as the budget needs) in
+        // two buffers, each chunk's stencil pass overlapping the previous
+        // chunk's pointwise pass.  SGR_PIPE_CHUNKS=k forces the pipelined
+        // schedule with at least k >= 2 chunks; SGR_PIPE_CHUNKS=0 forces
+        // consecutive chunks.
+        const uint64_t need = nt ? (nt + cap - 1) / cap : 1;
+        const char *pv = getenv("SGR_PIPE_CHUNKS");
+        const uint64_t pk = pv ? (uint64_t)std::max(0, atoi(pv)) : 0;
+        p->pipe = nt >= 2 && (pv ? pk >= 2 : need >= 2);
         // equal chunks (no short last launch)
-        const uint64_t nchunks = nt ? (nt + cap - 1) / cap : 1;
+        uint64_t nchunks = need;
+        if (p->pipe) nchunks = std::min<uint64_t>(nt, std::max<uint64_t>(2 * need, pv ? pk : PIPE_CHUNKS));
         p->chunk_tiles = (uint32_t)((nt + nchunks - 1) / nchunks);
-        p->ws_bytes = nt ? std::max<uint64_t>(p->chunk_tiles * rhs_tile, con_tile) : 0;
+        p->ws_bytes = nt ? std::max<uint64_t>((p->pipe ? 2 : 1) * p->chunk_tiles * rhs_tile, con_tile) : 0;
         p->cchunk_tiles = (uint32_t)std::min<uint64_t>(tile_block8.size(), p->ws_bytes / con_tile);
     }
     // optional cap on the plan's device memory (SGR_DEVICE_MEM_LIMIT_GIB):
@@ -486,6 +513,12 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         (e = dev_alloc((void **)&p->d_lam, 3 * p->lam_span * sizeof(double))) != hipSuccess)
         return bail(e, "hipMalloc(speeds)");
     p->grid_cap = resident_stencil_groups(device);
+    if (p->pipe) {
+        if ((e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking)) != hipSuccess)
+            return bail(e, "hipStreamCreate");
+        for (hipEvent_t &ev : p->pev)
+            if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return bail(e, "hipEventCreate");
+    }
     p->ghost = build_ghost_table(blocks, nblocks);
     if (!p->ghost.patches.empty()) {
         const size_t gb = p->ghost.patches.size() * sizeof(GhostPatch);
@@ -517,7 +550,47 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info)
     info->workspace_bytes = plan->ws_bytes;
     info->derivative_slots = rs_nslot(plan->nf) + plan->nf;
     info->chunks = plan->chunk_tiles ? (plan->ntiles + plan->chunk_tiles - 1) / plan->chunk_tiles : 0;
+    info->pipelined = plan->pipe && info->chunks > 1;
+    info->tile_queue = plan->use_queue;
     return SGR_OK;
+}
+
+// The chunks of a pipelined plan: chunk c uses workspace buffer c & 1; its
+// stencil pass runs on `stream` once the pointwise pass of chunk c - 2 (same
+// buffer) is done, its pointwise pass on plan->side once its stencil pass is.
+// Fork and join through events, so a captured graph records the same
+// dependencies.  events[0..1] bracket the whole call, events[2..3] are
+// recorded at its end.
+static hipError_t pipelined(sgr_plan *plan, LaunchArgs a, hipStream_t stream, hipEvent_t *ev)
+{
+    hipEvent_t *const E = plan->pev;  // fork, join, stencil done [2], pointwise done [2]
+    const uint64_t buf = plan->ws_bytes / 2;
+    const uint64_t wbytes = (uint64_t)plan->chunk_tiles * TILE_PTS * rs_nslot(plan->nf) * plan->wsz;
+    hipError_t e = ev ? hipEventRecord(ev[0], stream) : hipSuccess;
+    for (int q = 0; q < 4; q++) a.ev[q] = nullptr;
+    if (e == hipSuccess) e = hipEventRecord(E[0], stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(plan->side, E[0], 0);
+    uint32_t c = 0;
+    for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles; t0 += plan->chunk_tiles, c++) {
+        const int b = c & 1;
+        a.tile0 = t0;
+        a.ntiles = plan->ntiles - t0 < plan->chunk_tiles ? plan->ntiles - t0 : plan->chunk_tiles;
+        a.nfast = plan->nfast <= t0 ? 0 : std::min(plan->nfast - t0, a.ntiles);
+        a.ws = (char *)plan->d_ws + b * buf;
+        a.wsT = (double *)((char *)a.ws + wbytes);
+        if (c >= 2) e = hipStreamWaitEvent(stream, E[4 + b], 0);
+        a.only = 1;
+        if (e == hipSuccess) e = launch_rhs(a, stream);
+        if (e == hipSuccess) e = hipEventRecord(E[2 + b], stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(plan->side, E[2 + b], 0);
+        a.only = 2;
+        if (e == hipSuccess) e = launch_rhs(a, plan->side);
+        if (e == hipSuccess) e = hipEventRecord(E[4 + b], plan->side);
+    }
+    if (e == hipSuccess) e = hipEventRecord(E[1], plan->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, E[1], 0);
+    for (int q = 1; q < 4 && e == hipSuccess && ev; q++) e = hipEventRecord(ev[q], stream);
+    return e;
 }
 
 static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
@@ -561,6 +634,8 @@ static int rhs_impl(sgr_plan *plan, double *const out[SGR_NUM_VARS], const doubl
     hipError_t e = a.diss ? launch_speeds(a, stream) : hipSuccess;  // once, over every node
     if (e == hipSuccess && plan->ntiles <= plan->chunk_tiles) {
         e = launch_rhs(a, stream);
+    } else if (e == hipSuccess && plan->pipe) {
+        e = pipelined(plan, a, stream, ev);
     } else if (e == hipSuccess) {
         // several workspace chunks: events[0..1] bracket the whole call,
         // events[2..3] are recorded at its end

@@ -46,6 +46,7 @@ struct LaunchArgs {
     double diss_sigma;
     double *lam;                 // TVB: speeds along x, y, z, lam_span nodes each
     uint64_t lam_span;
+    int only;                    // RHS: 0 both passes, 1 the stencil pass only, 2 the rest only
 };
 
 // Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints; uses the workspace.

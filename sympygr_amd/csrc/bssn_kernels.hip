@@ -959,14 +959,14 @@ static hipError_t launch_t(const LaunchArgs &a, hipStream_t s)
     // hipExtLaunchKernelGGL records the optional events around exactly this
     // kernel on this stream (null events: plain launch).
     if (a.ccz4) {
-        hipError_t e = launch_stencil<W, UPWIND, NF_CCZ4>(a, ws, s);
-        if (e != hipSuccess) return e;
+        hipError_t e = a.only == 2 ? hipSuccess : launch_stencil<W, UPWIND, NF_CCZ4>(a, ws, s);
+        if (e != hipSuccess || a.only == 1) return e;
         hipExtLaunchKernelGGL((k_ccz4_pointwise<R, W, RK>), g, b, 0, s, a.ev[2], a.ev[3], 0, a.in, a.out, a.blocks,
                               a.tile_block, (const W *)ws, (const double *)a.wsT, a.params, a.rk, a.tile0);
         return hipGetLastError();
     }
-    hipError_t e = launch_stencil<W, UPWIND, NF>(a, ws, s);
-    if (e != hipSuccess) return e;
+    hipError_t e = a.only == 2 ? hipSuccess : launch_stencil<W, UPWIND, NF>(a, ws, s);
+    if (e != hipSuccess || a.only == 1) return e;
     if (a.diss) {  // TVB into the tails (BSSN only)
         if (a.diss == 2)
             hipLaunchKernelGGL(k_tvb<true>, g, b, 0, s, a.in, a.lam, a.lam_span, a.blocks, a.tile_block, a.wsT,
