@@ -292,9 +292,12 @@ def octree_section(args, hev):
         op = sgr.Plan(obl, sgr.params(adv="upwind", precision=prec, ko_sigma=0.1))
         outs[prec] = torch.zeros_like(ou)
         w, _, _ = timed_steps(op, outs[prec], ou, nst, 1, hev)
-        res[prec] = {"value": opts * nst / w / 1e6, "chunks": op.info().chunks, "ms_per_step": 1e3 * w / nst,
+        info = op.info()
+        res[prec] = {"value": opts * nst / w / 1e6, "chunks": info.chunks, "pipelined": bool(info.pipelined),
+                     "workspace_gib": info.workspace_bytes / 2 ** 30, "ms_per_step": 1e3 * w / nst,
                      "roofline": mix_roofline(size_pts, w / nst, prec)}
-        log(f"octree {prec}: {res[prec]['value']:.1f} Mpts/s in {op.info().chunks} workspace chunks")
+        log(f"octree {prec}: {res[prec]['value']:.1f} Mpts/s in {info.chunks} workspace chunks"
+            f"{' (pipelined)' if info.pipelined else ''}")
         del op
     err = normwise_err(outs["mixed"], outs["fp64"])
     ref = outs["fp64"]

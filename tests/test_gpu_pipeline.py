@@ -4,23 +4,26 @@ on the caller's stream overlapping the previous chunk's pointwise pass on a
 second stream.  Every point is computed by the same kernels in the same
 order, so results equal consecutive chunks and a single chunk bit for bit:
 plain RHS, RK4 stages, BSSN and CCZ4, both precisions, TVB dissipation, and
-a captured RK4 step replayed between eager work."""
+a captured RK4 step replayed between eager work.  A pipelined plan spreads
+blocks with face tiles over the tile order (SGR_TILE_ORDER forces either
+order); results do not depend on the order either."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _plan(bl, prm, monkeypatch, pipe=None, gib=None):
+def _plan(bl, prm, monkeypatch, pipe=None, gib=None, order=None):
     import sympygr_amd as sgr
-    for k, v in (("SGR_PIPE_CHUNKS", pipe), ("SGR_WS_GIB", gib)):
+    env = (("SGR_PIPE_CHUNKS", pipe), ("SGR_WS_GIB", gib), ("SGR_TILE_ORDER", order))
+    for k, v in env:
         if v is None:
             monkeypatch.delenv(k, raising=False)
         else:
             monkeypatch.setenv(k, v)
     plan = sgr.Plan(bl, prm)
-    monkeypatch.delenv("SGR_PIPE_CHUNKS", raising=False)
-    monkeypatch.delenv("SGR_WS_GIB", raising=False)
+    for k, _ in env:
+        monkeypatch.delenv(k, raising=False)
     return plan
 
 
@@ -42,9 +45,10 @@ def test_pipelined_equals_consecutive_and_single_chunk(kind, monkeypatch):
         u = data.fill_blocks(bl, backend="torch", device="cuda", kind="perturbed", seed=6, amp=1e-2, lo=-1.0, L=2.0)
     prm = sgr.params(**kw)
     res = {}
-    for name, pipe, gib in (("single", None, None), ("consecutive", "0", "0.01"), ("pipelined", "5", None),
-                            ("budget", None, "0.01")):
-        plan = _plan(bl, prm, monkeypatch, pipe, gib)
+    for name, pipe, gib, order in (("single", None, None, None), ("consecutive", "0", "0.01", None),
+                                   ("pipelined", "5", None, None), ("budget", None, "0.01", None),
+                                   ("single spread", None, None, "spread"), ("pipelined block", "5", None, "block")):
+        plan = _plan(bl, prm, monkeypatch, pipe, gib, order)
         info = plan.info()
         print(f"{kind} {name}: chunks={info.chunks} pipelined={info.pipelined}", flush=True)
         out = torch.full_like(u, 5.0)
@@ -60,7 +64,7 @@ def test_pipelined_equals_consecutive_and_single_chunk(kind, monkeypatch):
     assert res["pipelined"][3] >= 5 and res["pipelined"][4]
     # a workspace budget that needs several chunks selects the pipelined schedule
     assert res["budget"][4] and res["budget"][3] >= 16
-    for name in ("consecutive", "pipelined", "budget"):
+    for name in ("consecutive", "pipelined", "budget", "single spread", "pipelined block"):
         for k in range(3):
             assert torch.equal(res[name][k], res["single"][k]), (kind, name, k)
 
