@@ -432,6 +432,26 @@ def interior_blocks_leg(bl, u, steps, hev):
     return res
 
 
+def tvb_leg(bl, u, steps, hev):
+    """The 512-block workload with the TVB dissipation the reference's
+    bssnrhs_sep selects with DISSIPATION_TYPE 1 and 2 (rhs.cpp:729-757)
+    instead of Kreiss-Oliger, FP64: the speeds kernel and k_tvb run beside
+    the two RHS passes (whole-call rate)."""
+    import torch
+    import sympygr_amd as sgr
+    pts = bl.interior_points()
+    res = {"workload": "the 512 x 32^3 blocks, FP64, upwind, sigma 0.1, TVB3 / TVB5 dissipation"}
+    out = torch.zeros_like(u)
+    for diss in ("tvb3", "tvb5"):
+        plan = sgr.Plan(bl, sgr.params(adv="upwind", precision="fp64", ko_sigma=0.1, dissipation=diss))
+        w, _, _ = timed_steps(plan, out, u, steps, 2, hev)
+        res[diss] = {"value": pts * steps / w / 1e6, "ms_per_step": 1e3 * w / steps}
+        log(f"{diss} dissipation fp64: {res[diss]['value']:.1f} Mpts/s, {res[diss]['ms_per_step']:.3f} ms per RHS")
+        del plan
+    del out
+    return res
+
+
 def dendro_sizes_section(hev, sizes=(9, 13, 17, 25, 33), steps=10):
     """Block sizes of Dendro's own meshes (2^k p + 1 interior points): one
     RHS in FP64 over a uniform tiling of about 12 M interior points per
@@ -608,6 +628,7 @@ def main():
     ap.add_argument("--no-octree", action="store_true",
                     help="skip the batched uneven-octree measurements (RHS, RK4 step, drift) and the large block")
     ap.add_argument("--no-ccz4", action="store_true", help="skip the CCZ4 192^3-block leg")
+    ap.add_argument("--no-tvb", action="store_true", help="skip the TVB-dissipation leg")
     ap.add_argument("--no-dendro-sizes", action="store_true",
                     help="skip the leg over Dendro's block sizes (9/13/17/25/33, tuned tiles against 8^3)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -677,6 +698,9 @@ def main():
     interior = None
     if not args.counters:
         interior = interior_blocks_leg(bl, u, args.steps, hev)
+    tvb = None
+    if not (args.counters or args.no_tvb):
+        tvb = tvb_leg(bl, u, max(2, args.steps // 2), hev)
 
     rhs_ms = kd + kp
     abytes = alg_bytes_per_point(n) * pts
@@ -801,6 +825,7 @@ def main():
                    "blocks": len(bl), "interior_points": pts, "adv": "upwind", "precision": prec0},
         "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "batched_octree": octree,
         "constraint_drift": drift, "large_block": large, "ccz4_large_block": ccz4, "interior_blocks": interior,
+        "tvb_dissipation": tvb,
         "dendro_block_sizes": dendro,
         "host_dropin": host,
     }
