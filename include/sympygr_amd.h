@@ -118,8 +118,9 @@ typedef struct {
     uint32_t chunks;             /* launches per pass (workspace budget SGR_WS_GIB) */
     uint32_t pipelined;          /* 1: the chunks run pipelined (two workspace buffers; each
                                     chunk's stencil pass overlaps the previous chunk's
-                                    pointwise pass on a second stream the plan owns), as
-                                    every plan needing more than one chunk does */
+                                    pointwise pass on a second stream the plan owns): plans
+                                    needing more than one chunk, and plans of at least 8192
+                                    tiles without the tile queues */
     uint32_t tile_queue;         /* 1: the stencil pass is a persistent grid fed by per-XCD
                                     tile queues (plans where at least a tenth of the tiles
                                     reach a flagged face) */
@@ -142,11 +143,12 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info);
 int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
             hipStream_t stream);
 
-/* Same as sgr_rhs, and records events[0..1] around the derivative kernel and
+/* Same as sgr_rhs, and records events[0..1] around the stencil pass and
  * events[2..3] around the pointwise kernel on `stream` (hipEventElapsedTime
  * then gives each pass's duration).  For a plan that runs in several
- * workspace chunks, events[0..1] bracket the whole call and events[2..3] are
- * recorded at its end.  No reference counterpart: it exists for measurement. */
+ * workspace chunks (pipelined or not), events[0..1] bracket the whole call
+ * and events[2..3] are recorded at its end.  No reference counterpart: it
+ * exists for measurement. */
 int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
                    hipStream_t stream, hipEvent_t events[4]);
 

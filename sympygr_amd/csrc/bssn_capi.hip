@@ -60,9 +60,15 @@ AxisConst axis_const(double pmin, double pmax, uint32_t n)
     return a;
 }
 
-// Chunks of a pipelined plan, at least (the octree of 3032 leaves ran 10-12%
-// faster in FP64 with 16 pipelined chunks than with its 3 consecutive ones).
+// The pipelined schedule: up to PIPE_CHUNKS chunks of at least
+// PIPE_MIN_TILES tiles, or as many as the workspace budget needs.  Measured:
+// the 3032-leaf octree (233,738 tiles) ran 10-12% faster with 16 pipelined
+// chunks than with its 3 consecutive ones and 4-8% faster than in one chunk;
+// the 216 interior-only 32^3 blocks (13,824 tiles) 2-3% faster in 4 chunks
+// than in one, but slower in 8 (mixed precision); chunks of 2048 tiles
+// (512 x 32^3 in 16) lost up to 18%.
 constexpr uint64_t PIPE_CHUNKS = 16;
+constexpr uint64_t PIPE_MIN_TILES = 4096;
 
 // Workspace bytes per 8^3 tile: RHS pass (staged derivatives in W plus the
 // double tails) and constraint pass (its derivative slots plus KO sums).
@@ -422,11 +428,13 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
             p->lam_span = std::max<uint64_t>(p->lam_span, blocks[b].offset + (uint64_t)blocks[b].sz[0] *
                                                                                blocks[b].sz[1] * blocks[b].sz[2]);
     }
-    uint64_t nt_all = 0, nt8_all = 0;  // RHS and constraint tiles
+    uint64_t nt_all = 0, nt8_all = 0, faces = 0;  // RHS and constraint tiles, RHS tiles at a flagged face
     for (const BlockDesc &d : p->hblocks) {
         nt_all += (uint64_t)d.ntile[0] * d.ntile[1] * d.ntile[2];
         nt8_all += (uint64_t)d.ntile8[0] * d.ntile8[1] * d.ntile8[2];
+        faces += face_tiles(d);
     }
+    p->use_queue = stencil_queue(faces, nt_all);
     // The workspace holds at most `chunk_tiles` tiles of the RHS pass (budget
     // SGR_WS_GIB, default 40 GiB); larger plans run as several launches over
     // equal tile ranges.  The constraint pass stages more per tile and runs
@@ -441,19 +449,24 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
         uint64_t cap = (uint64_t)(gib * 1073741824.0) / rhs_tile;
         if (cap < 1) cap = 1;
         const uint64_t nt = nt_all;
-        // A plan whose workspace takes more than one chunk runs pipelined:
-        // at least PIPE_CHUNKS chunks (twice as many as the budget needs) in
-        // two buffers, each chunk's stencil pass overlapping the previous
-        // chunk's pointwise pass.  SGR_PIPE_CHUNKS=k forces the pipelined
-        // schedule with at least k >= 2 chunks; SGR_PIPE_CHUNKS=0 forces
-        // consecutive chunks.
+        // A plan runs pipelined (two workspace buffers, each chunk's stencil
+        // pass overlapping the previous chunk's pointwise pass on a second
+        // stream) when its workspace takes more than one chunk, or when it
+        // has at least 2 PIPE_MIN_TILES tiles and one workgroup per stencil
+        // tile (the persistent tile-queue grid holds every CU and leaves the
+        // other stream nothing to overlap).  Chunks: twice as many as the
+        // budget needs, or up to PIPE_CHUNKS of at least PIPE_MIN_TILES
+        // tiles.  SGR_PIPE_CHUNKS=k forces the pipelined schedule with at
+        // least k >= 2 chunks; SGR_PIPE_CHUNKS=0 forces consecutive chunks.
         const uint64_t need = nt ? (nt + cap - 1) / cap : 1;
         const char *pv = getenv("SGR_PIPE_CHUNKS");
         const uint64_t pk = pv ? (uint64_t)std::max(0, atoi(pv)) : 0;
-        p->pipe = nt >= 2 && (pv ? pk >= 2 : need >= 2);
+        p->pipe = nt >= 2 && (pv ? pk >= 2 : need >= 2 || (!p->use_queue && nt >= 2 * PIPE_MIN_TILES));
         // equal chunks (no short last launch)
         uint64_t nchunks = need;
-        if (p->pipe) nchunks = std::min<uint64_t>(nt, std::max<uint64_t>(2 * need, pv ? pk : PIPE_CHUNKS));
+        if (p->pipe)
+            nchunks = std::min<uint64_t>(
+                nt, std::max<uint64_t>({2 * need, pv ? pk : std::min(PIPE_CHUNKS, nt / PIPE_MIN_TILES), 2}));
         p->chunk_tiles = (uint32_t)((nt + nchunks - 1) / nchunks);
         p->ws_bytes = nt ? std::max<uint64_t>((p->pipe ? 2 : 1) * p->chunk_tiles * rhs_tile, con_tile) : 0;
         p->cchunk_tiles = (uint32_t)std::min<uint64_t>(nt8_all, p->ws_bytes / con_tile);
@@ -489,11 +502,6 @@ int sgr_plan_create(sgr_plan **plan, const sgr_block *blocks, uint32_t nblocks, 
     }
     p->ntiles = (uint32_t)tile_block.size();
     p->ntiles8 = (uint32_t)tile_block8.size();
-    {
-        uint64_t faces = 0;
-        for (const BlockDesc &d : p->hblocks) faces += face_tiles(d);
-        p->use_queue = stencil_queue(faces, p->ntiles);
-    }
     // optional cap on the plan's device memory (SGR_DEVICE_MEM_LIMIT_GIB):
     // a plan that would need more is refused before anything is allocated
     {

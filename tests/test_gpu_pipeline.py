@@ -62,8 +62,9 @@ def test_pipelined_equals_consecutive_and_single_chunk(kind, monkeypatch):
     assert res["single"][3] == 1 and not res["single"][4]
     assert res["consecutive"][3] > 1 and not res["consecutive"][4]
     assert res["pipelined"][3] >= 5 and res["pipelined"][4]
-    # a workspace budget that needs several chunks selects the pipelined schedule
-    assert res["budget"][4] and res["budget"][3] >= 16
+    # a workspace budget that needs several chunks selects the pipelined
+    # schedule, with twice as many chunks
+    assert res["budget"][4] and res["budget"][3] >= 2 * res["consecutive"][3] - 1
     for name in ("consecutive", "pipelined", "budget", "single spread", "pipelined block"):
         for k in range(3):
             assert torch.equal(res[name][k], res["single"][k]), (kind, name, k)
@@ -105,3 +106,29 @@ def test_pipelined_graph_replays_match_eager(monkeypatch):
         piped.rk4_step(dt, dp, *bufs_p)
     torch.cuda.synchronize()
     assert torch.equal(dp, de)
+
+
+def test_large_plan_without_tile_queues_runs_pipelined(monkeypatch):
+    """A plan of at least 8192 tiles whose stencil pass runs one workgroup
+    per tile (no flagged faces, so no tile queues) runs pipelined in one
+    workspace budget, and equals the consecutive schedule bit for bit."""
+    import torch
+    import sympygr_amd as sgr
+    from sympygr_amd import data
+    bl = data.uniform_tiling(nb=8, n=24, lo=-1.0, hi=1.0, stagger=0.5, faces=False)
+    u = data.fill_blocks(bl, backend="torch", device="cuda", kind="perturbed", seed=2, amp=1e-2, lo=-1.0, L=2.0)
+    prm = sgr.params(adv="upwind", ko_sigma=0.1)
+    outs = []
+    for pipe in (None, "0"):
+        plan = _plan(bl, prm, monkeypatch, pipe)
+        info = plan.info()
+        print(f"SGR_PIPE_CHUNKS={pipe}: tiles={info.tiles} chunks={info.chunks} pipelined={info.pipelined} "
+              f"queue={info.tile_queue}", flush=True)
+        assert info.tile_queue == 0 and info.tiles >= 8192
+        assert bool(info.pipelined) == (pipe is None)
+        out = torch.full_like(u, 5.0)
+        plan.rhs(out, u)
+        torch.cuda.synchronize()
+        outs.append(out)
+        del plan
+    assert torch.equal(outs[0], outs[1])
