@@ -137,9 +137,13 @@ int sgr_plan_get_info(const sgr_plan *plan, sgr_plan_info *info);
 
 /* RHS of every block of the plan on `stream`; in/out are DEVICE pointers to
  * the 24 unzipped arrays.  Asynchronous; allocates nothing, so it can be
- * captured into a HIP graph.  Replaces the reference's multi-block entry
- * bssnRHS (bssn/include/rhs.h:31) and its GPU plug-in cuda::computeRHS
- * (bssn/cuda_gr/include/rhs_cuda.cuh:39). */
+ * captured into a HIP graph.  A pipelined plan (sgr_plan_info) also runs
+ * work on its own second stream, forked from and joined back into `stream`
+ * through events, so the call is ordered on `stream` as one launch would
+ * be.  Calls on one plan share its workspace (and second stream) and must
+ * not overlap: issue them on one stream or serialise them.  Replaces the
+ * reference's multi-block entry bssnRHS (bssn/include/rhs.h:31) and its GPU
+ * plug-in cuda::computeRHS (bssn/cuda_gr/include/rhs_cuda.cuh:39). */
 int sgr_rhs(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double *const in[SGR_NUM_VARS],
             hipStream_t stream);
 
