@@ -391,6 +391,15 @@ def ccz4_section(hev, steps=10):
             torch.cuda.synchronize()
             tc = (time.perf_counter() - t0) / 3
             res["constraints"] = {"ms": 1e3 * tc, "mpts": pts / tc / 1e6}
+            c = torch.zeros((6, bl.total), dtype=torch.float64, device="cuda")
+            plan.constraints_psi4(u, c)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                plan.constraints_psi4(u, c)
+            torch.cuda.synchronize()
+            tc = (time.perf_counter() - t0) / 3
+            res["constraints_psi4"] = {"ms": 1e3 * tc, "mpts": pts / tc / 1e6}
             del c
         del plan
     err = normwise_err(outs["mixed"], outs["fp64"])

@@ -167,6 +167,19 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
  * (bssn/CodeGen/physcon.cpp) does not compile. */
 int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream);
 
+/* Same as sgr_constraints, and also the real (out[4]) and imaginary (out[5])
+ * parts of the Weyl scalar Psi4 at every interior point (GR/constraints.py:
+ * 46-155): tetrad built by Gram-Schmidt from the coordinate vectors r, theta,
+ * phi at the node's coordinates (x, y, z) = pmin + (i, j, k) h of its block
+ * (dendrosym/codegen.py:732).  Keeps the statement of GR/constraints.py as
+ * written, including the sign of the half dchi terms in the fourth part of
+ * the imaginary part (GR/constraints.py:150), so on a boosted slice of flat
+ * space the imaginary part is not zero where those terms are not.  The
+ * reference's counterpart is the generated physcon kernel that
+ * sgr_constraints replaces. */
+int sgr_constraints_psi4(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[6],
+                         hipStream_t stream);
+
 /* Ghost refresh (the unzip step between RK stages), in place.  Every block
  * owns the box of its interior nodes widened by half a cell; a ghost node
  * inside another block's owned box takes that block's value: the coinciding

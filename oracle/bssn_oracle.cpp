@@ -531,22 +531,27 @@ extern "C" int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t n
     return err ? -1 : 0;
 }
 
+namespace {
+
 // Hamiltonian and momentum constraints at the interior points of one block
-// (GR/constraints.py:161-173), from the same derivative sweeps.
-extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t offset, const double *pmin,
-                                  const double *pmax, const uint32_t *sz, uint32_t bflag)
+// (GR/constraints.py:161-173), from the same derivative sweeps; with psi4 also
+// the real and imaginary parts of the Weyl scalar Psi4 (GR/constraints.py:46-155)
+// at the point's coordinates pmin + i h.
+int constraints_impl(double **outv, const double **inv, uint64_t offset, const double *pmin, const double *pmax,
+                     const uint32_t *sz, uint32_t bflag, bool psi4)
 {
     if (!outv || !inv || !pmin || !pmax || !sz) return -1;
     if (sz[0] < 13 || sz[1] < 13 || sz[2] < 13 || bflag > 63) return -2;
     const Grid g{(int)sz[0], (int)sz[1], (int)sz[2]};
     const std::size_t n = (std::size_t)g.nx * g.ny * g.nz;
+    const int nout = psi4 ? 6 : 4;
     const double *in[24];
-    double *out[4];
+    double *out[6];
     for (int f = 0; f < 24; f++) {
         if (!inv[f]) return -1;
         in[f] = inv[f] + offset;
     }
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < nout; c++) {
         if (!outv[c]) return -1;
         out[c] = outv[c] + offset;
     }
@@ -565,7 +570,12 @@ extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t of
 #define BSSN_LAP(f) metric_laplacian(in, W.grad2[f], pp)
 #define BSSN_OUT(c, v) out[c][pp] = (v)
 #define BSSN_BARRIER()
+                if (psi4) {
+                    const R px = pmin[0] + i * h[0], py = pmin[1] + j * h[1], pz = pmin[2] + k * h[2];
+#include "gen/bssn_constraints_psi4.inc"
+                } else {
 #include "gen/bssn_constraints.inc"
+                }
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
@@ -574,6 +584,20 @@ extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t of
 #undef BSSN_BARRIER
             }
     return 0;
+}
+
+}  // namespace
+
+extern "C" int oracle_constraints(double **outv, const double **inv, uint64_t offset, const double *pmin,
+                                  const double *pmax, const uint32_t *sz, uint32_t bflag)
+{
+    return constraints_impl(outv, inv, offset, pmin, pmax, sz, bflag, false);
+}
+
+extern "C" int oracle_constraints_psi4(double **outv, const double **inv, uint64_t offset, const double *pmin,
+                                       const double *pmax, const uint32_t *sz, uint32_t bflag)
+{
+    return constraints_impl(outv, inv, offset, pmin, pmax, sz, bflag, true);
 }
 
 extern "C" int oracle_deriv(int which, double *out, const double *u, double h, const uint32_t *sz, const double *beta,

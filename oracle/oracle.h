@@ -55,6 +55,11 @@ int oracle_bssnrhs_blocks(double **out, const double **in, uint32_t nblocks,
 int oracle_constraints(double **out, const double **in, uint64_t offset, const double *pmin, const double *pmax,
                        const uint32_t *sz, uint32_t bflag);
 
+/* Same, and the real (out[4]) and imaginary (out[5]) parts of Psi4 at the
+ * coordinates pmin + (i, j, k) h, GR/constraints.py:46-155. */
+int oracle_constraints_psi4(double **out, const double **in, uint64_t offset, const double *pmin,
+                            const double *pmax, const uint32_t *sz, uint32_t bflag);
+
 /* Ghost refresh of a block set in place (oracle/ghost_oracle.cpp): every
  * ghost node owned by another block's interior box takes that block's value,
  * coinciding node or 6-point Lagrange interpolant per axis.  -1 when the

@@ -78,6 +78,7 @@ def lib():
                                             ctypes.POINTER(OracleParams), ctypes.c_int]
         L.oracle_constraints.argtypes = [ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_uint64, P_D, P_D,
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]
+        L.oracle_constraints_psi4.argtypes = L.oracle_constraints.argtypes
         L.oracle_ghost_fill.argtypes = [ctypes.POINTER(P_D), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.POINTER(ctypes.c_uint32), P_D, P_D, ctypes.POINTER(ctypes.c_uint32)]
         L.oracle_deriv.argtypes = [ctypes.c_int, P_D, P_D, ctypes.c_double, ctypes.POINTER(ctypes.c_uint32), P_D,
@@ -120,12 +121,14 @@ def rhs_blocks(u, bl, params, nthreads=1, out=None):
     return out
 
 
-def constraints_block(u, sz, pmin, pmax, bflag):
-    """(4, npts): Hamiltonian and momentum constraints at interior points."""
+def constraints_block(u, sz, pmin, pmax, bflag, psi4=False):
+    """(4, npts): Hamiltonian and momentum constraints at interior points;
+    psi4=True: (6, npts), with Re/Im Psi4 in rows 4 and 5."""
     u = np.ascontiguousarray(u, dtype=np.float64)
-    out = np.zeros((4, u.shape[1]))
-    rc = lib().oracle_constraints(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin),
-                                  (ctypes.c_double * 3)(*pmax), (ctypes.c_uint32 * 3)(*sz), bflag)
+    out = np.zeros((6 if psi4 else 4, u.shape[1]))
+    fn = lib().oracle_constraints_psi4 if psi4 else lib().oracle_constraints
+    rc = fn(_ptrs(out), _ptrs(u), 0, (ctypes.c_double * 3)(*pmin), (ctypes.c_double * 3)(*pmax),
+            (ctypes.c_uint32 * 3)(*sz), bflag)
     if rc != 0:
         raise RuntimeError(f"oracle_constraints failed: {rc}")
     return out

@@ -104,6 +104,8 @@ _SIGS = {
     "sgr_rhs_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p,
                                       ctypes.POINTER(ctypes.c_void_p)]),
     "sgr_constraints": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D), ctypes.c_void_p]),
+    "sgr_constraints_psi4": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.POINTER(P_D),
+                                            ctypes.c_void_p]),
     "sgr_ghost_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(P_D), ctypes.c_void_p]),
     "sgr_hbm_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
     "sgr_ghost_table_stats": (ctypes.c_int, [ctypes.POINTER(SgrBlock), ctypes.c_uint32,
@@ -290,6 +292,14 @@ class Plan:
         interior points; out4 is a (4, total) float64 tensor."""
         po = _rows(out4, 4, self.blocks.total, self.device)
         check(lib().sgr_constraints(self._h, self._f(u), po, ctypes.c_void_p(self._stream(u, stream))))
+
+    def constraints_psi4(self, u, out6, stream=None):
+        """Constraints (out6[0:4], as constraints()) and the real (out6[4])
+        and imaginary (out6[5]) parts of Psi4 at the interior points, at the
+        coordinates pmin + i h of each block; out6 is a (6, total) float64
+        tensor."""
+        po = _rows(out6, 6, self.blocks.total, self.device)
+        check(lib().sgr_constraints_psi4(self._h, self._f(u), po, ctypes.c_void_p(self._stream(u, stream))))
 
     def ghost_fill(self, u, stream=None):
         """Ghost refresh of u in place (copies where grids align, 6-point

@@ -347,7 +347,7 @@ def bssn_rhs_stages(contract_beta: bool = True, tails: bool = False) -> Stages:
     return S
 
 
-def constraint_stages() -> Stages:
+def constraint_stages(psi4: bool = False, parts: bool = False) -> Stages:
     """Hamiltonian and momentum constraints, GR/constraints.py:161-173, with the
     Ricci tensor of GR/dendro.py:468-527 (the same staged pieces as the RHS):
 
@@ -355,7 +355,9 @@ def constraint_stages() -> Stages:
         M_i = igt^{jk} (d_k A_ij - C2[m,k,i] A_jm) - Gt^j A_ij
               - 3/2 igt^{jk} A_ki d_j chi / chi - 2/3 d_i K
 
-    Outputs are named ham, mom0..2 and stored with BSSN_OUT(0..3, v)."""
+    Outputs are named ham, mom0..2 and stored with BSSN_OUT(0..3, v).  With
+    psi4, also the real and imaginary parts of the Weyl scalar Psi4
+    (GR/constraints.py:46-155; psi4_stages), BSSN_OUT(4..5, v)."""
     R = bssn_rhs_stages()
     keep = {"idet", "igt", "inv_chi", "C1", "C2", "igt_dchi", "CalGt", "C1u", "Rt", "Rphi_tr", "Rphi",
             "Amix", "AtU"}
@@ -385,4 +387,118 @@ def constraint_stages() -> Stages:
         m -= sp.Rational(3, 2) * sum(igt[j][k] * At[k][i] * d1("chi", j) * inv_chi for (j, k) in E33)
         m -= sp.Rational(2, 3) * d1("K", i)
         S.outputs.append((f"mom{i}", m))
+    if psi4:
+        psi4_stages(S, parts)
     return S
+
+
+# 1/sqrt(2) as the reference writes it (GR/constraints.py:55)
+INV_SQRT2 = sp.Float("0.7071067811865475244", 30)
+
+
+def psi4_stages(S: Stages, parts: bool = False) -> None:
+    """The Weyl scalar Psi4 for wave extraction, GR/constraints.py:46-155,
+    appended to a constraint Stages (whose igt, inv_chi, C2 and Ricci stages
+    it reads) as outputs psi4_real, psi4_img.
+
+    Tetrad: the coordinate vectors r = (x, y, z), theta = (x z, y z,
+    -(x^2 + y^2)), phi = (-y, x, 0) at the point (px, py, pz, plain variables
+    in scope), made orthonormal in that order by Gram-Schmidt with the
+    physical metric gt / chi; m = (theta + i phi) / sqrt(2).  With
+    A_i = A~_ij r^j, MM = Re(m m), NN = Im(m m), MR / NR the antisymmetric
+    products of Re m / Im m with r, and U_ij (V_ij) = (Re m)^k ((Im m)^k)
+    (d_j A~_ki + C2^m_ki A~_mj):
+
+        Re Psi4 =   R.MM + a A~.MM - (MR.U - NR.V)/chi - (...)/chi^2
+        Im Psi4 = -(R.NN + a A~.NN - (NR.U + MR.V)/chi - (...)/chi^2)
+
+    with a = (A_i r^i + K chi/3 + 1/2 r.d chi) / chi^2, R the Ricci tensor of
+    the physical metric and the last terms as GR/constraints.py:149-150."""
+    I = Inputs()
+    u = I.u
+    d1 = I.d1
+    chi, K = u["chi"], u["K"]
+    Atn = [[f"At{sym_slot(i, j)}" for j in E3] for i in E3]
+    At = [[u[Atn[i][j]] for j in E3] for i in E3]
+    gt = [[u[f"gt{sym_slot(i, j)}"] for j in E3] for i in E3]
+    C2 = [[[sp.Symbol(f"C2_{i}_{min(j, k)}{max(j, k)}") for k in E3] for j in E3] for i in E3]
+    Rt = _sym3("Rt")
+    Rphi = _sym3("Rphi")
+    inv_chi = sp.Symbol("inv_chi")
+    px, py, pz = sp.Symbol("px"), sp.Symbol("py"), sp.Symbol("pz")
+    half = sp.Rational(1, 2)
+
+    def dot(a, b):  # physical inner product gt_ij / chi a^i b^j
+        return inv_chi * sum(gt[i][j] * a[i] * b[j] for (i, j) in E33)
+
+    # Gram-Schmidt, r first, then theta, then phi (GR/constraints.py:57-92)
+    rv = [px, py, pz]
+    n_r = sp.Symbol("psi4_nr")
+    S.stage("psi4_nr", [(n_r, 1 / sp.sqrt(dot(rv, rv)))])
+    rn = [sp.Symbol(f"psi4_r{i}") for i in E3]
+    S.stage("psi4_r", [(rn[i], rv[i] * n_r) for i in E3])
+    th = [px * pz, py * pz, -(px * px + py * py)]
+    t1, t2 = sp.Symbol("psi4_tt"), sp.Symbol("psi4_rt")
+    S.stage("psi4_th_ip", [(t1, dot(th, th)), (t2, dot(rn, th))])
+    n_t = sp.Symbol("psi4_nt")
+    S.stage("psi4_nt", [(n_t, 1 / sp.sqrt(t1 - t2 * t2))])
+    tn = [sp.Symbol(f"psi4_t{i}") for i in E3]
+    S.stage("psi4_t", [(tn[i], (th[i] - t2 * rn[i]) * n_t) for i in E3])
+    ph = [-py, px, sp.Integer(0)]
+    q1, q2, q3 = sp.Symbol("psi4_pp"), sp.Symbol("psi4_rp"), sp.Symbol("psi4_tp")
+    S.stage("psi4_ph_ip", [(q1, dot(ph, ph)), (q2, dot(rn, ph)), (q3, dot(tn, ph))])
+    n_p = sp.Symbol("psi4_np")
+    S.stage("psi4_np", [(n_p, 1 / sp.sqrt(q1 - q2 * q2 - q3 * q3))])
+    pn = [sp.Symbol(f"psi4_p{i}") for i in E3]
+    S.stage("psi4_p", [(pn[i], (ph[i] - q2 * rn[i] - q3 * tn[i]) * n_p) for i in E3])
+
+    # the complex null vector m = (theta + i phi)/sqrt(2) and its products
+    mr = [tn[i] * INV_SQRT2 for i in E3]
+    mi = [pn[i] * INV_SQRT2 for i in E3]
+    MM = [[sp.Symbol(f"psi4_MM{sym_slot(i, j)}") for j in E3] for i in E3]
+    NN = [[sp.Symbol(f"psi4_NN{sym_slot(i, j)}") for j in E3] for i in E3]
+    S.stage("psi4_MMNN", [(MM[i][j], mr[i] * mr[j] - mi[i] * mi[j]) for (i, j) in SYM] +
+            [(NN[i][j], mr[i] * mi[j] + mr[j] * mi[i]) for (i, j) in SYM])
+
+    def contract_sym(T, W):  # T_ij W^ij over a symmetric pair, diagonal + 2 x upper
+        return sum(T[i][i] * W[i][i] for i in E3) + 2 * sum(T[i][j] * W[i][j] for i in E3 for j in E3 if j > i)
+
+    Avec = [sp.Symbol(f"psi4_A{i}") for i in E3]
+    S.stage("psi4_A", [(Avec[i], sum(At[i][j] * rn[j] for j in E3)) for i in E3])
+    # U_ij, V_ij (GR/constraints.py:125-128)
+    wpairs = []
+    Wt = [[[None] * 3 for _ in E3] for _ in E3]
+    for k in E3:
+        for i in E3:
+            for j in E3:
+                s = sp.Symbol(f"psi4_W{k}{i}{j}")
+                wpairs.append((s, d1(Atn[k][i], j) + sum(C2[m][k][i] * At[m][j] for m in E3)))
+                Wt[k][i][j] = s
+    S.stage("psi4_W", wpairs)
+    U = [[sp.Symbol(f"psi4_U{i}{j}") for j in E3] for i in E3]
+    V = [[sp.Symbol(f"psi4_V{i}{j}") for j in E3] for i in E3]
+    S.stage("psi4_UV", [(U[i][j], sum(mr[k] * Wt[k][i][j] for k in E3)) for i in E3 for j in E3] +
+            [(V[i][j], sum(mi[k] * Wt[k][i][j] for k in E3)) for i in E3 for j in E3])
+    rdchi = sum(rn[i] * d1("chi", i) for i in E3)
+    atmp = sp.Symbol("psi4_a")
+    S.stage("psi4_a", [(atmp, inv_chi * inv_chi * (sum(Avec[i] * rn[i] for i in E3) + K * chi / 3 + half * rdchi))])
+    mr_dchi = sum(mr[i] * d1("chi", i) for i in E3)
+    mi_dchi = sum(mi[i] * d1("chi", i) for i in E3)
+    mr_A = sum(mr[i] * Avec[i] for i in E3)
+    mi_A = sum(mi[i] * Avec[i] for i in E3)
+    R = [[Rt[i][j] + Rphi[i][j] for j in E3] for i in E3]
+    MR = [[mr[i] * rn[j] - mr[j] * rn[i] for j in E3] for i in E3]
+    NR = [[mi[i] * rn[j] - mi[j] * rn[i] for j in E3] for i in E3]
+    p1r, p1i = contract_sym(R, MM), contract_sym(R, NN)
+    p2r, p2i = atmp * contract_sym(At, MM), atmp * contract_sym(At, NN)
+    p3r = inv_chi * sum(MR[i][j] * U[i][j] - NR[i][j] * V[i][j] for (i, j) in E33)
+    p3i = inv_chi * sum(NR[i][j] * U[i][j] + MR[i][j] * V[i][j] for (i, j) in E33)
+    p4r = inv_chi * inv_chi * (mr_A * (mr_A + half * mr_dchi) - mi_A * (mi_A + half * mi_dchi))
+    p4i = inv_chi * inv_chi * (mr_A * (mi_A - half * mi_dchi) + mi_A * (mr_A - half * mr_dchi))
+    S.outputs.append(("psi4_real", p1r + p2r - p3r - p4r))
+    S.outputs.append(("psi4_img", -(p1i + p2i - p3i - p4i)))
+    if parts:
+        for n, e in (("p1r", p1r), ("p2r", p2r), ("p3r", p3r), ("p4r", p4r),
+                     ("p1i", p1i), ("p2i", p2i), ("p3i", p3i), ("p4i", p4i),
+                     ("mrA", mr_A), ("miA", mi_A), ("mrd", mr_dchi), ("mid", mi_dchi), ("ic", inv_chi)):
+            S.outputs.append((f"psi4_{n}", e))

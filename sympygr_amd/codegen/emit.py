@@ -79,6 +79,9 @@ class _Printer(C99CodePrinter):
 
     def _print_Pow(self, expr):
         b, e = expr.as_base_exp()
+        if e == sp.Rational(1, 2) or e == sp.Rational(-1, 2):  # the Psi4 tetrad's norms
+            root = f"sqrt({self._print(b)})"
+            return root if e > 0 else f"(R(1)/{root})"
         if not e.is_Integer:
             raise ValueError(f"non-integer power in algebra: {expr}")
         n = int(e)

@@ -722,7 +722,8 @@ int sgr_rhs_events(sgr_plan *plan, double *const out[SGR_NUM_VARS], const double
     return rhs_impl(plan, out, in, stream, events);
 }
 
-int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream)
+static int constraints_impl(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const *out, bool psi4,
+                     hipStream_t stream)
 {
     if (!plan || !in || !out) return fail(SGR_E_INVALID, "null argument");
     LaunchArgs a{};
@@ -731,7 +732,7 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
         if (!in[f]) return fail(SGR_E_INVALID, "null field pointer");
         a.in.p[f] = in[f];
     }
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < (psi4 ? 6 : 4); c++) {
         if (!out[c]) return fail(SGR_E_INVALID, "null constraint pointer");
         co.p[c] = out[c];
     }
@@ -750,11 +751,22 @@ int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double
     for (uint32_t t0 = 0; e == hipSuccess && t0 < plan->ntiles8; t0 += plan->cchunk_tiles) {
         a.tile0 = t0;
         a.ntiles = plan->ntiles8 - t0 < plan->cchunk_tiles ? plan->ntiles8 - t0 : plan->cchunk_tiles;
-        e = launch_constraints(a, co, stream);
+        e = launch_constraints(a, co, psi4, stream);
     }
     if (cur != plan->device) hipSetDevice(cur);
     if (e != hipSuccess) return hip_fail(e, "constraint kernels");
     return SGR_OK;
+}
+
+int sgr_constraints(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[4], hipStream_t stream)
+{
+    return constraints_impl(plan, in, out, false, stream);
+}
+
+int sgr_constraints_psi4(sgr_plan *plan, const double *const in[SGR_NUM_VARS], double *const out[6],
+                         hipStream_t stream)
+{
+    return constraints_impl(plan, in, out, true, stream);
 }
 
 int sgr_ghost_fill(sgr_plan *plan, double *const u[SGR_NUM_VARS], hipStream_t stream)

@@ -49,8 +49,9 @@ struct LaunchArgs {
     int only;                    // RHS: 0 both passes, 1 the stencil pass only, 2 the rest only
 };
 
-// Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints; uses the workspace.
-hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s);
+// Hamiltonian (cout[0]) and momentum (cout[1..3]) constraints, with psi4 also
+// Re/Im Psi4 (cout[4], cout[5]); uses the workspace.
+hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, bool psi4, hipStream_t s);
 
 // boxes sorted by class: class c occupies [bstart[c], bstart[c+1])
 hipError_t launch_ghost_fill(OutPtrs u, int nf, const GhostPatch *patches, const GhostBox *boxes,

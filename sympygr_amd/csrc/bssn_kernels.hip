@@ -1001,8 +1001,10 @@ hipError_t launch_rhs(const LaunchArgs &a, hipStream_t s)
 }
 
 // Hamiltonian and momentum constraints (GR/constraints.py:161-173) at every
-// interior point, in double, from the derivative workspace.
-template <typename W, bool UPWIND>
+// interior point, in double, from the derivative workspace; with PSI4 also the
+// real and imaginary parts of the Weyl scalar Psi4 (GR/constraints.py:46-155)
+// at the point's coordinates pmin + i h (dendrosym/codegen.py:732).
+template <typename W, bool UPWIND, bool PSI4>
 __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout, const BlockDesc *__restrict__ blocks,
                                                      const uint32_t *__restrict__ tile_block, const W *__restrict__ ws,
                                                      uint32_t tile0)
@@ -1027,7 +1029,12 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
 #define BSSN_GDB(J_) ((R)ws_get(&wsb[slot_gdb(J_) * SS]))
 #define BSSN_OUT(C_, V_) cout.p[C_][pp] = (V_)
 #define BSSN_BARRIER()
+    if constexpr (PSI4) {
+        const R px = B.pmin[0] + i * B.ax[0].h, py = B.pmin[1] + j * B.ax[1].h, pz = B.pmin[2] + k * B.ax[2].h;
+#include "gen/bssn_constraints_psi4.inc"
+    } else {
 #include "gen/bssn_constraints.inc"
+    }
 #undef BSSN_U
 #undef BSSN_D1
 #undef BSSN_D2
@@ -1037,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_constraints(FieldPtrs in, OutPtrs cout,
 }
 
 template <typename W, bool UPWIND>
-static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
+static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, bool psi4, hipStream_t s)
 {
     W *ws = (W *)a.ws;
     if (a.derivs_threads == 512)
@@ -1048,16 +1055,23 @@ static hipError_t launch_constraints_t(const LaunchArgs &a, OutPtrs cout, hipStr
                            a.wsK, true, a.tile0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_constraints<W, UPWIND>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
-                       a.tile_block, (const W *)ws, a.tile0);
+    if (psi4)
+        hipLaunchKernelGGL((k_constraints<W, UPWIND, true>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout, a.blocks,
+                           a.tile_block, (const W *)ws, a.tile0);
+    else
+        hipLaunchKernelGGL((k_constraints<W, UPWIND, false>), dim3(2 * a.ntiles), dim3(256), 0, s, a.in, cout,
+                           a.blocks, a.tile_block, (const W *)ws, a.tile0);
     return hipGetLastError();
 }
 
-hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, hipStream_t s)
+hipError_t launch_constraints(const LaunchArgs &a, OutPtrs cout, bool psi4, hipStream_t s)
 {
     if (a.ntiles == 0) return hipSuccess;
-    if (a.mixed) return a.upwind ? launch_constraints_t<float, true>(a, cout, s) : launch_constraints_t<float, false>(a, cout, s);
-    return a.upwind ? launch_constraints_t<double, true>(a, cout, s) : launch_constraints_t<double, false>(a, cout, s);
+    if (a.mixed)
+        return a.upwind ? launch_constraints_t<float, true>(a, cout, psi4, s)
+                        : launch_constraints_t<float, false>(a, cout, psi4, s);
+    return a.upwind ? launch_constraints_t<double, true>(a, cout, psi4, s)
+                    : launch_constraints_t<double, false>(a, cout, psi4, s);
 }
 
 // Taps of one axis for target node `idx`: returns the first source node and
